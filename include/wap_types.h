@@ -1,0 +1,58 @@
+/* Plain-data types shared by the wap C ABI (include/wap.h).
+ *
+ * Memory layout everywhere is planar x-fastest, the layout of the volume images the
+ * warp-and-project loop works on: voxel (x, y, z, c) of a W x H x D x C field lives at
+ * x + W * (y + H * (z + D * c)). Detector images are u-fastest: pixel (u, v) at
+ * u + Nu * v.
+ */
+#ifndef WAP_TYPES_H
+#define WAP_TYPES_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Scan description used to build per-view matrices. The cone-beam form follows the
+ * reference camera: Scene::_calculateCamera / _calculateMatrix
+ * (Project/Scene.cpp:65-189), TypicalRayProducer::_calculateMatrix
+ * (Project/TypicalRayProducer.cpp:157-232), PSART ctor
+ * (Project/WaP_Tomo.cpp:228-256) and PSART::RotateVolume (Project/WaP_Tomo.cpp:1764-1797).
+ * The parallel-beam form (parallel = 1) is an extension with the same rotation and
+ * detector conventions and rays along the view axis. */
+typedef struct WapScanGeometry {
+  int vol_dims[3];       /* W, H, D voxels */
+  float vol_spacing;     /* voxel pitch (reference: spacings[1]) */
+  int det_u, det_v;      /* detector pixels Nu, Nv */
+  double det_pitch;      /* detector pixel pitch (reference: m_ds) */
+  float sid;             /* source to rotation centre (reference: m_sid) */
+  float sdd;             /* source to detector (reference: m_sdd, also the near plane) */
+  float translate[3];    /* view translation (reference: m_Translate) */
+  int counterclockwise;  /* reference IS_COUNTERCLOCK */
+  int parallel;          /* 0 = cone beam (reference), 1 = parallel beam */
+  int pad_;
+} WapScanGeometry;
+
+/* Everything a projector or backprojector kernel needs for one view. All matrices are
+ * column-major 4x4 floats (ele[col*4 + row]) exactly as the reference Matrix4x4. */
+typedef struct WapView {
+  float p2m[16];    /* pixel -> model (cone: inverse of ViewToPixels*Proj*ModelView;
+                       parallel: pixel -> point on the source plane) */
+  float m2v[16];    /* model -> voxel */
+  float mvv2m[16];  /* ModelView * VolumeToModel, used by the voxel-driven backprojector */
+  float eye[4];     /* cone: source position in model space (P2M column 2 / P2M[11]) */
+  float dir[4];     /* parallel: unit ray direction in model space */
+  double half_w;    /* (Nu-1)*ds/2 (reference m_halfFrustumWidth) */
+  double half_h;    /* (Nv-1)*ds/2 */
+  double ds;        /* detector pitch */
+  float sdd;        /* source-detector distance */
+  int parallel;     /* 0 cone, 1 parallel */
+} WapView;
+
+/* Storage element type of volumes, flows and sinograms. */
+typedef enum WapDType { WAP_F32 = 0, WAP_BF16 = 1, WAP_F16 = 2 } WapDType;
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* WAP_TYPES_H */

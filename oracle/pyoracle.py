@@ -1,0 +1,390 @@
+"""ctypes front-end of the CPU oracle (oracle/liboracle.so) and of the reference-compiled
+checker (oracle/_ref/libwapref.so) -- TEST INFRASTRUCTURE ONLY.
+
+Imported by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg; the product
+path never imports it. Arrays are numpy, planar x-fastest (index x + W*(y + H*(z + D*c))),
+which numpy spells as shape (C, D, H, W) in C order.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_SO = os.path.join(HERE, "liboracle.so")
+REF_SO = os.path.join(HERE, "_ref", "libwapref.so")
+
+f32p = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+
+
+class WapScanGeometry(C.Structure):
+    _fields_ = [("vol_dims", C.c_int * 3), ("vol_spacing", C.c_float),
+                ("det_u", C.c_int), ("det_v", C.c_int), ("det_pitch", C.c_double),
+                ("sid", C.c_float), ("sdd", C.c_float), ("translate", C.c_float * 3),
+                ("counterclockwise", C.c_int), ("parallel", C.c_int), ("pad_", C.c_int)]
+
+
+class WapView(C.Structure):
+    _fields_ = [("p2m", C.c_float * 16), ("m2v", C.c_float * 16), ("mvv2m", C.c_float * 16),
+                ("eye", C.c_float * 4), ("dir", C.c_float * 4), ("half_w", C.c_double),
+                ("half_h", C.c_double), ("ds", C.c_double), ("sdd", C.c_float),
+                ("parallel", C.c_int)]
+
+
+assert C.sizeof(WapScanGeometry) == 64 and C.sizeof(WapView) == 256
+
+
+class OrcLoop(C.Structure):
+    _fields_ = [("dims", C.c_int * 3), ("nstates", C.c_int), ("nviews", C.c_int),
+                ("nu", C.c_int), ("nv", C.c_int), ("sd", C.c_float), ("alpha", C.c_double),
+                ("gmax", C.c_float), ("nb_sart", C.c_int), ("roi", C.c_int * 8),
+                ("cut", C.c_int * 8), ("flow_tol", C.c_float), ("flow_lambda", C.c_float),
+                ("flow_eta", C.c_float), ("flow_sigma", C.c_float), ("flow_iters", C.c_int),
+                ("flow_warps", C.c_int), ("flow_scales", C.c_int), ("do_flow", C.c_int),
+                ("do_warp", C.c_int), ("timeunit", C.c_float)]
+
+
+def make_geometry(dims, nu, nv, ds=1.0, sid=400.0, sdd=600.0, spacing=1.0, parallel=False,
+                  ccw=True, translate=(0.0, 0.0, 0.0)):
+    g = WapScanGeometry()
+    g.vol_dims[:] = list(dims)
+    g.vol_spacing = spacing
+    g.det_u, g.det_v = nu, nv
+    g.det_pitch = ds
+    g.sid, g.sdd = sid, sdd
+    g.translate[:] = list(translate)
+    g.counterclockwise = 1 if ccw else 0
+    g.parallel = 1 if parallel else 0
+    return g
+
+
+def _dims(shape):
+    """numpy (D,H,W) or (C,D,H,W) -> int32 [W,H,D]."""
+    d, h, w = shape[-3:]
+    return np.array([w, h, d], dtype=np.int32)
+
+
+def _c(a, dt):
+    return np.ascontiguousarray(a, dtype=dt)
+
+
+class Oracle:
+    def __init__(self, path=ORACLE_SO):
+        if not os.path.exists(path):
+            raise FileNotFoundError(f"oracle library missing: {path} (run make -C oracle)")
+        L = self.lib = C.CDLL(path)
+        vp = C.POINTER(WapView)
+        gp = C.POINTER(WapScanGeometry)
+        L.orc_view_geometry.argtypes = [gp, C.c_double, vp]
+        L.orc_view_order.argtypes = [C.c_int, C.c_int, i32p]
+        L.orc_frame_windows.argtypes = [C.c_int, C.c_int, C.c_float, f32p, i32p, i32p]
+        L.orc_roi_mask.argtypes = [i32p, i32p, f32p]
+        L.orc_forward_project_f32.argtypes = [f32p, i32p, C.c_float, vp, C.c_int, C.c_int,
+                                              f32p, f32p, f32p]
+        L.orc_forward_project_f64.argtypes = [f64p, i32p, C.c_double, vp, C.c_int, C.c_int,
+                                              f64p, f64p, f64p]
+        L.orc_produce_rays.argtypes = [i32p, vp, C.c_int, C.c_int, f32p, f32p, f32p, f32p]
+        L.orc_backproject_ray_f32.argtypes = [f32p, i32p, C.c_float, vp, C.c_int, C.c_int,
+                                              C.c_float, f32p, f32p]
+        L.orc_backproject_ray_f64.argtypes = [f64p, i32p, C.c_double, vp, C.c_int, C.c_int,
+                                              C.c_double, f64p, f64p]
+        L.orc_sart_correct_f32.argtypes = [f32p, f32p, f32p, f32p, C.c_double, f32p, C.c_int]
+        L.orc_backproject_voxel_f32.argtypes = [f32p, i32p, f32p, vp, C.c_int, C.c_int,
+                                                C.c_double, C.c_float, i32p, i32p, C.c_int]
+        L.orc_backproject_voxel_f64.argtypes = [f64p, i32p, f64p, vp, C.c_int, C.c_int,
+                                                C.c_double, C.c_double, i32p, i32p, C.c_int]
+        L.orc_advect_f32.argtypes = [i32p, f32p, f32p, f32p, C.c_int, C.c_float, f32p]
+        L.orc_advect_f64.argtypes = [i32p, f64p, f64p, f64p, C.c_int, C.c_double, f64p]
+        L.orc_gradient_f32.argtypes = [f32p, i32p, f32p]
+        L.orc_blur_f32.argtypes = [f32p, i32p, C.c_int, C.c_float, f32p]
+        L.orc_blur_f64.argtypes = [f64p, i32p, C.c_int, C.c_double, f64p]
+        L.orc_resize_f32.argtypes = [f32p, i32p, C.c_int, i32p, f32p]
+        L.orc_resize_f64.argtypes = [f64p, i32p, C.c_int, i32p, f64p]
+        L.orc_flow_single_f32.argtypes = [i32p, f32p, f32p, f32p, f32p, f32p, f32p, f32p,
+                                          C.c_float, C.c_float, C.c_int, C.c_int, f32p, i32p,
+                                          C.POINTER(C.c_int)]
+        L.orc_flow_multiscale_f32.argtypes = [i32p, f32p, f32p, C.c_void_p, C.c_float,
+                                              C.c_float, C.c_int, C.c_int, C.c_float,
+                                              C.c_float, C.c_int, i32p, f32p]
+        L.orc_flow_multiscale_f64.argtypes = [i32p, f64p, f64p, C.c_void_p, C.c_double,
+                                              C.c_double, C.c_int, C.c_int, C.c_double,
+                                              C.c_double, C.c_int, i32p, f64p]
+        L.orc_tvtp_step_f32.argtypes = [C.c_int, i32p, i32p, f32p, f32p, f32p, f32p, f32p,
+                                        f32p, f32p, C.c_float, C.c_float, C.c_float,
+                                        C.c_float, C.c_float]
+        L.orc_xbar_update_f32.argtypes = [i32p, i32p, f32p, f32p, f32p]
+        lp = C.POINTER(OrcLoop)
+        L.orc_wap_iteration_f32.argtypes = [lp, C.c_void_p, f32p, i32p, i32p, f32p, f32p,
+                                            f32p, f32p]
+        L.orc_wap_iteration_f64.argtypes = [lp, C.c_void_p, f64p, i32p, i32p, f32p, f64p,
+                                            f64p, f64p]
+
+    # -- geometry -------------------------------------------------------------------
+    def view(self, geom, angle_deg):
+        v = WapView()
+        self.lib.orc_view_geometry(C.byref(geom), float(angle_deg), C.byref(v))
+        return v
+
+    def views(self, geom, angles_deg):
+        arr = (WapView * len(angles_deg))()
+        for i, a in enumerate(angles_deg):
+            self.lib.orc_view_geometry(C.byref(geom), float(a), C.byref(arr[i]))
+        return arr
+
+    def view_order(self, first, last):
+        out = np.zeros(last - first + 1, np.int32)
+        self.lib.orc_view_order(first, last, out)
+        return out
+
+    def frame_windows(self, nframes, total_views, base=3.0):
+        ft = np.zeros(nframes, np.float32)
+        s = np.zeros(nframes, np.int32)
+        e = np.zeros(nframes, np.int32)
+        self.lib.orc_frame_windows(nframes, total_views, base, ft, s, e)
+        return ft, s, e
+
+    def roi_mask(self, shape, roi8):
+        m = np.zeros(shape, np.float32)
+        self.lib.orc_roi_mask(_dims(shape), _c(roi8, np.int32), m)
+        return m
+
+    # -- projector ------------------------------------------------------------------
+    def forward_project(self, vol, view, nu, nv, sd, f64=False):
+        if f64:
+            vol = _c(vol, np.float64)
+            out = [np.zeros((nv, nu), np.float64) for _ in range(3)]
+            self.lib.orc_forward_project_f64(vol, _dims(vol.shape), sd, C.byref(view), nu, nv,
+                                             *out)
+        else:
+            vol = _c(vol, np.float32)
+            out = [np.zeros((nv, nu), np.float32) for _ in range(3)]
+            self.lib.orc_forward_project_f32(vol, _dims(vol.shape), sd, C.byref(view), nu, nv,
+                                             *out)
+        return tuple(out)
+
+    def produce_rays(self, shape, view, nu, nv):
+        z = np.zeros((nv * nu, 4), np.float32)
+        u = np.zeros((nv * nu, 4), np.float32)
+        tn = np.zeros(nv * nu, np.float32)
+        tf = np.zeros(nv * nu, np.float32)
+        self.lib.orc_produce_rays(_dims(shape), C.byref(view), nu, nv, z, u, tn, tf)
+        return z, u, tn, tf
+
+    def backproject_ray(self, corr, shape, view, sd, scale=1.0, f64=False):
+        nv, nu = corr.shape
+        dt = np.float64 if f64 else np.float32
+        out = np.zeros(shape, dt)
+        ws = np.zeros(shape, dt)
+        fn = self.lib.orc_backproject_ray_f64 if f64 else self.lib.orc_backproject_ray_f32
+        fn(_c(corr, dt), _dims(shape), sd, C.byref(view), nu, nv, scale, out, ws)
+        return out, ws
+
+    def sart_correct(self, b, ax, raylen, py, alpha):
+        corr = np.zeros_like(ax, dtype=np.float32)
+        py = _c(py, np.float32).copy()
+        self.lib.orc_sart_correct_f32(_c(b, np.float32), _c(ax, np.float32),
+                                      _c(raylen, np.float32), py, alpha, corr, corr.size)
+        return corr, py
+
+    def backproject_voxel(self, vol, corr, view, alpha, gmax, lo, hi, mode, f64=False):
+        nv, nu = corr.shape
+        dt = np.float64 if f64 else np.float32
+        vol = _c(vol, dt).copy()
+        fn = self.lib.orc_backproject_voxel_f64 if f64 else self.lib.orc_backproject_voxel_f32
+        fn(vol, _dims(vol.shape), _c(corr, dt), C.byref(view), nu, nv, alpha, gmax,
+           _c(lo, np.int32), _c(hi, np.int32), mode)
+        return vol
+
+    # -- advection / image ops ------------------------------------------------------
+    def advect(self, vel, src, dt, aabb=None, f64=False):
+        d = _dims(src.shape)
+        nc = 1 if src.ndim == 3 else src.shape[0]
+        if aabb is None:
+            aabb = [0, d[0], 0, d[1], 0, d[2]]
+        typ = np.float64 if f64 else np.float32
+        out = np.zeros(src.shape, typ)
+        fn = self.lib.orc_advect_f64 if f64 else self.lib.orc_advect_f32
+        fn(d, _c(aabb, typ), _c(vel, typ), _c(src, typ), nc, dt, out)
+        return out
+
+    def gradient(self, img):
+        out = np.zeros((3,) + img.shape, np.float32)
+        self.lib.orc_gradient_f32(_c(img, np.float32), _dims(img.shape), out)
+        return out
+
+    def blur(self, img, sigma, f64=False):
+        nc = 1 if img.ndim == 3 else img.shape[0]
+        typ = np.float64 if f64 else np.float32
+        out = np.zeros(img.shape, typ)
+        fn = self.lib.orc_blur_f64 if f64 else self.lib.orc_blur_f32
+        fn(_c(img, typ), _dims(img.shape), nc, sigma, out)
+        return out
+
+    def resize(self, img, new_dhw, f64=False):
+        nc = 1 if img.ndim == 3 else img.shape[0]
+        typ = np.float64 if f64 else np.float32
+        shape = tuple(new_dhw) if img.ndim == 3 else (nc,) + tuple(new_dhw)
+        out = np.zeros(shape, typ)
+        nd = np.array([new_dhw[2], new_dhw[1], new_dhw[0]], np.int32)
+        fn = self.lib.orc_resize_f64 if f64 else self.lib.orc_resize_f32
+        fn(_c(img, typ), _dims(img.shape), nc, nd, out)
+        return out
+
+    # -- flow ----------------------------------------------------------------------
+    def flow_single(self, I1, I2, V, Y1, Y2, Y3, Y4, tol, lam, max_iter, nwarps, aabb, roi):
+        V, Y1, Y2, Y3, Y4 = [_c(a, np.float32).copy() for a in (V, Y1, Y2, Y3, Y4)]
+        it = C.c_int(0)
+        self.lib.orc_flow_single_f32(_dims(I1.shape), _c(I1, np.float32), _c(I2, np.float32),
+                                     V, Y1, Y2, Y3, Y4, tol, lam, max_iter, nwarps,
+                                     _c(aabb, np.float32), _c(roi, np.int32), C.byref(it))
+        return V, Y1, Y2, Y3, Y4, it.value
+
+    def flow_multiscale(self, I1, I2, V_init, tol, lam, max_iter, nwarps, eta, sigma, scales,
+                        cut, f64=False):
+        typ = np.float64 if f64 else np.float32
+        out = np.zeros((3,) + I1.shape, typ)
+        vi = None
+        if V_init is not None:
+            V_init = _c(V_init, typ)
+            vi = V_init.ctypes.data
+        fn = self.lib.orc_flow_multiscale_f64 if f64 else self.lib.orc_flow_multiscale_f32
+        fn(_dims(I1.shape), _c(I1, typ), _c(I2, typ), vi, tol, lam, max_iter, nwarps, eta,
+           sigma, scales, _c(cut, np.int32), out)
+        return out
+
+    def wap_iteration(self, loop, views, sino, ws, we, ftime, vols, flows, mask, f64=False):
+        typ = np.float64 if f64 else np.float32
+        vols = _c(vols, typ).copy()
+        flows = _c(flows, typ).copy()
+        fn = self.lib.orc_wap_iteration_f64 if f64 else self.lib.orc_wap_iteration_f32
+        fn(C.byref(loop), C.addressof(views), _c(sino, typ), _c(ws, np.int32),
+           _c(we, np.int32), _c(ftime, np.float32), vols, flows, _c(mask, typ))
+        return vols, flows
+
+
+class Ref:
+    """The reference's own code compiled from its sources (oracle/_ref/libwapref.so)."""
+
+    def __init__(self, path=REF_SO):
+        if not os.path.exists(path):
+            raise FileNotFoundError(path)
+        L = self.lib = C.CDLL(path)
+        L.ref_project_view.argtypes = [f32p, i32p, f32p, f32p, C.c_float, C.c_int, C.c_int,
+                                       f32p, f32p, f32p]
+        L.ref_produce_rays.argtypes = [i32p, f32p, f32p, C.c_int, C.c_int, f32p, f32p, f32p,
+                                       f32p]
+        L.ref_cast_rays.argtypes = [f32p, i32p, C.c_float, C.c_int, f32p, f32p, f32p, f32p,
+                                    f32p, f32p]
+        L.ref_scatter_rays.argtypes = [i32p, C.c_float, C.c_int, f32p, f32p, f32p, f32p, f32p,
+                                       f32p, f32p]
+        L.ref_advect.argtypes = [i32p, f32p, f32p, f32p, C.c_int, C.c_float, f32p]
+        L.ref_blur.argtypes = [f32p, i32p, C.c_int, C.c_float, f32p]
+        L.ref_resize.argtypes = [f32p, i32p, C.c_int, i32p, C.c_int, f32p]
+        L.ref_gradient.argtypes = [f32p, i32p, f32p]
+        L.ref_cut.argtypes = [f32p, i32p, C.c_float, C.c_float]
+        L.ref_flow_single.argtypes = [i32p, f32p, f32p, f32p, f32p, f32p, f32p, f32p,
+                                      C.c_float, C.c_float, C.c_int, C.c_int, f32p, i32p]
+        L.ref_flow_multiscale.argtypes = [i32p, f32p, f32p, C.c_void_p, C.c_float, C.c_float,
+                                          C.c_int, C.c_int, C.c_float, C.c_float, C.c_int,
+                                          i32p, f32p]
+        L.ref_matrix_mul.argtypes = [f32p, f32p, f32p]
+        L.ref_matrix_inverse.argtypes = [f32p, f32p]
+        L.ref_matrix_inverse.restype = C.c_float
+        L.ref_matrix_vec.argtypes = [f32p, f32p, f32p]
+        L.ref_quaternion_matrix.argtypes = [C.c_float, C.c_float, C.c_float, C.c_float, f32p]
+        L.ref_translate_matrix.argtypes = [C.c_float, C.c_float, C.c_float, f32p]
+        L.ref_scale_matrix.argtypes = [C.c_float, C.c_float, C.c_float, f32p]
+
+    def project_view(self, vol, view, nu, nv, sd):
+        vol = _c(vol, np.float32)
+        out = [np.zeros((nv, nu), np.float32) for _ in range(3)]
+        self.lib.ref_project_view(vol, _dims(vol.shape), np.array(view.p2m, np.float32),
+                                  np.array(view.m2v, np.float32), sd, nu, nv, *out)
+        return tuple(out)
+
+    def produce_rays(self, shape, view, nu, nv):
+        z = np.zeros((nv * nu, 4), np.float32)
+        u = np.zeros((nv * nu, 4), np.float32)
+        tn = np.zeros(nv * nu, np.float32)
+        tf = np.zeros(nv * nu, np.float32)
+        self.lib.ref_produce_rays(_dims(shape), np.array(view.p2m, np.float32),
+                                  np.array(view.m2v, np.float32), nu, nv, z, u, tn, tf)
+        return z, u, tn, tf
+
+    def cast_rays(self, vol, sd, zero4, unit4, tn, tf):
+        vol = _c(vol, np.float32)
+        n = len(tn)
+        out = np.zeros(n, np.float32)
+        w = np.zeros(n, np.float32)
+        self.lib.ref_cast_rays(vol, _dims(vol.shape), sd, n, _c(zero4, np.float32),
+                               _c(unit4, np.float32), _c(tn, np.float32), _c(tf, np.float32),
+                               out, w)
+        return out, w
+
+    def scatter_rays(self, shape, sd, zero4, unit4, tn, tf, corr):
+        n = len(tn)
+        c = np.zeros(shape, np.float32)
+        w = np.zeros(shape, np.float32)
+        self.lib.ref_scatter_rays(_dims(shape), sd, n, _c(zero4, np.float32),
+                                  _c(unit4, np.float32), _c(tn, np.float32), _c(tf, np.float32),
+                                  _c(corr, np.float32), c, w)
+        return c, w
+
+    def advect(self, vel, src, dt, aabb=None):
+        d = _dims(src.shape)
+        nc = 1 if src.ndim == 3 else src.shape[0]
+        if aabb is None:
+            aabb = [0, d[0], 0, d[1], 0, d[2]]
+        out = np.zeros(src.shape, np.float32)
+        self.lib.ref_advect(d, _c(aabb, np.float32), _c(vel, np.float32), _c(src, np.float32),
+                            nc, dt, out)
+        return out
+
+    def blur(self, img, sigma):
+        nc = 1 if img.ndim == 3 else img.shape[0]
+        out = np.zeros(img.shape, np.float32)
+        self.lib.ref_blur(_c(img, np.float32), _dims(img.shape), nc, sigma, out)
+        return out
+
+    def resize(self, img, new_dhw, interp=5):
+        nc = 1 if img.ndim == 3 else img.shape[0]
+        shape = tuple(new_dhw) if img.ndim == 3 else (nc,) + tuple(new_dhw)
+        out = np.zeros(shape, np.float32)
+        nd = np.array([new_dhw[2], new_dhw[1], new_dhw[0]], np.int32)
+        self.lib.ref_resize(_c(img, np.float32), _dims(img.shape), nc, nd, interp, out)
+        return out
+
+    def gradient(self, img):
+        out = np.zeros((3,) + img.shape, np.float32)
+        self.lib.ref_gradient(_c(img, np.float32), _dims(img.shape), out)
+        return out
+
+    def flow_single(self, I1, I2, V, Y1, Y2, Y3, Y4, tol, lam, max_iter, nwarps, aabb, roi):
+        V, Y1, Y2, Y3, Y4 = [_c(a, np.float32).copy() for a in (V, Y1, Y2, Y3, Y4)]
+        self.lib.ref_flow_single(_dims(I1.shape), _c(I1, np.float32), _c(I2, np.float32),
+                                 V, Y1, Y2, Y3, Y4, tol, lam, max_iter, nwarps,
+                                 _c(aabb, np.float32), _c(roi, np.int32))
+        return V, Y1, Y2, Y3, Y4
+
+    def flow_multiscale(self, I1, I2, V_init, tol, lam, max_iter, nwarps, eta, sigma, scales,
+                        cut):
+        out = np.zeros((3,) + I1.shape, np.float32)
+        vi = None
+        if V_init is not None:
+            V_init = _c(V_init, np.float32)
+            vi = V_init.ctypes.data
+        self.lib.ref_flow_multiscale(_dims(I1.shape), _c(I1, np.float32), _c(I2, np.float32),
+                                     vi, tol, lam, max_iter, nwarps, eta, sigma, scales,
+                                     _c(cut, np.int32), out)
+        return out
+
+    def matrix(self, name, *args):
+        out = np.zeros(16, np.float32)
+        getattr(self.lib, name)(*args, out)
+        return out
+
+
+def available_ref():
+    return os.path.exists(REF_SO)

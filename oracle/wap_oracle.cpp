@@ -1,0 +1,1356 @@
+// CPU oracle for the warp-and-project loop -- TEST INFRASTRUCTURE ONLY.
+//
+// This is the checker the HIP product path is compared against; it is never part of
+// the product. Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg
+// load it (through oracle/liboracle.so).
+//
+// Every routine restates the reference's arithmetic from its source (file:line cited
+// per function, paths relative to the reference root). Templates take the value type
+// R: R = float gives the parity form, which reproduces the reference's float/double
+// mix operation by operation (built with -ffp-contract=off, so a*b+c is never fused);
+// R = double gives the FP64 yardstick. Where the reference evaluates a sub-expression
+// in double (double literals, double members), the f32 form does the same.
+//
+// Pinning (see tests/): geometry, projector, scatter, advection, CImg-equivalent
+// image ops and the optical-flow solver are compared against the reference's own
+// code compiled from its sources (oracle/_ref) and against committed golden vectors;
+// the SART correction, voxel-driven bicubic backprojector, view ordering, frame
+// windows, ROI mask and TV/TP step live in files that cannot be compiled here
+// (Project/WaP_Tomo.cpp needs GL/Windows headers) and are pinned by closed forms
+// (constant and linear fields, Catmull-Rom reproduction of linear data, brute-force
+// recomputation on tiny inputs and the permutation/window identities).
+
+#include "wap_oracle.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+namespace {
+
+// ---------------------------------------------------------------------------------
+// 4x4 column-major float matrices, arithmetic as Project/Matrix4x4.{h,cpp}
+// ---------------------------------------------------------------------------------
+struct M4 {
+  float e[16];
+};
+
+M4 m4_identity() {
+  M4 m;
+  for (int i = 0; i < 16; ++i) m.e[i] = (i % 5 == 0) ? 1.0f : 0.0f;
+  return m;
+}
+
+// Project/Matrix4x4.h:66-84 (operator*)
+M4 m4_mul(const M4& A, const M4& B) {
+  M4 r;
+  for (int c = 0; c < 4; ++c)
+    for (int rr = 0; rr < 4; ++rr)
+      r.e[4 * c + rr] = A.e[rr] * B.e[4 * c] + A.e[4 + rr] * B.e[4 * c + 1] +
+                        A.e[8 + rr] * B.e[4 * c + 2] + A.e[12 + rr] * B.e[4 * c + 3];
+  return r;
+}
+
+inline float det3(float a, float b, float c, float d, float e, float f, float g, float h,
+                  float i) {
+  return a * e * i + b * f * g + c * d * h - c * e * g - a * f * h - b * d * i;
+}
+
+// Project/Matrix4x4.cpp:80-135 (cofactor inverse in float)
+M4 m4_inverse(const M4& m) {
+  const float a1 = m.e[0], b1 = m.e[1], c1 = m.e[2], d1 = m.e[3];
+  const float a2 = m.e[4], b2 = m.e[5], c2 = m.e[6], d2 = m.e[7];
+  const float a3 = m.e[8], b3 = m.e[9], c3 = m.e[10], d3 = m.e[11];
+  const float a4 = m.e[12], b4 = m.e[13], c4 = m.e[14], d4 = m.e[15];
+  const float det1 = det3(b2, b3, b4, c2, c3, c4, d2, d3, d4);
+  const float det2 = det3(a2, a3, a4, c2, c3, c4, d2, d3, d4);
+  const float det3v = det3(a2, a3, a4, b2, b3, b4, d2, d3, d4);
+  const float det4 = det3(a2, a3, a4, b2, b3, b4, c2, c3, c4);
+  const float det = a1 * det1 - b1 * det2 + c1 * det3v - d1 * det4;
+  M4 r = m;
+  if (det == 0.0f) return r;
+  const float inv = 1.0f / det;
+  r.e[0] = det1 * inv;
+  r.e[4] = -det2 * inv;
+  r.e[8] = det3v * inv;
+  r.e[12] = -det4 * inv;
+  r.e[1] = -det3(b1, b3, b4, c1, c3, c4, d1, d3, d4) * inv;
+  r.e[5] = det3(a1, a3, a4, c1, c3, c4, d1, d3, d4) * inv;
+  r.e[9] = -det3(a1, a3, a4, b1, b3, b4, d1, d3, d4) * inv;
+  r.e[13] = det3(a1, a3, a4, b1, b3, b4, c1, c3, c4) * inv;
+  r.e[2] = det3(b1, b2, b4, c1, c2, c4, d1, d2, d4) * inv;
+  r.e[6] = -det3(a1, a2, a4, c1, c2, c4, d1, d2, d4) * inv;
+  r.e[10] = det3(a1, a2, a4, b1, b2, b4, d1, d2, d4) * inv;
+  r.e[14] = -det3(a1, a2, a4, b1, b2, b4, c1, c2, c4) * inv;
+  r.e[3] = -det3(b1, b2, b3, c1, c2, c3, d1, d2, d3) * inv;
+  r.e[7] = det3(a1, a2, a3, c1, c2, c3, d1, d2, d3) * inv;
+  r.e[11] = -det3(a1, a2, a3, b1, b2, b3, d1, d2, d3) * inv;
+  r.e[15] = det3(a1, a2, a3, b1, b2, b3, c1, c2, c3) * inv;
+  return r;
+}
+
+M4 m4_translate(float x, float y, float z) {
+  M4 m = m4_identity();
+  m.e[12] = x;
+  m.e[13] = y;
+  m.e[14] = z;
+  return m;
+}
+
+M4 m4_scale(float x, float y, float z) {
+  M4 m = m4_identity();
+  m.e[0] = x;
+  m.e[5] = y;
+  m.e[10] = z;
+  return m;
+}
+
+// Project/Quaternion.cpp:7-40 (axis-angle ctor), Quaternion.h:46-86 (ToMatrix, product)
+M4 quat_matrix(float ax, float ay, float az, float angle) {
+  float x, y, z, w;
+  const float length = sqrtf(ax * ax + ay * ay + az * az);
+  if (0.0f == length) {
+    w = 1.0f;
+    x = y = z = 0.0f;
+  } else {
+    const float inv = 1.0f / length;
+    const float xx = ax * inv, yy = ay * inv, zz = az * inv;
+    const float rad = angle / 2;
+    w = cosf(rad);
+    const float s = sinf(rad);
+    x = xx * s;
+    y = yy * s;
+    z = zz * s;
+  }
+  // p * identity (m_Rotation = Quaternion(...) * m_OldRotation, identity by default)
+  const float qw = 1.0f, qx = 0.0f, qy = 0.0f, qz = 0.0f;
+  const float rw = w * qw - x * qx - y * qy - z * qz;
+  const float rx = w * qx + x * qw + y * qz - z * qy;
+  const float ry = w * qy + y * qw + z * qx - x * qz;
+  const float rz = w * qz + z * qw + x * qy - y * qx;
+  x = rx;
+  y = ry;
+  z = rz;
+  w = rw;
+  M4 m;
+  float* e = m.e;
+  e[0] = 1.0f - 2.0f * (y * y + z * z);
+  e[1] = 2.0f * (x * y + z * w);
+  e[2] = 2.0f * (z * x - y * w);
+  e[3] = 0.0f;
+  e[4] = 2.0f * (x * y - z * w);
+  e[5] = 1.0f - 2.0f * (z * z + x * x);
+  e[6] = 2.0f * (y * z + x * w);
+  e[7] = 0.0f;
+  e[8] = 2.0f * (z * x + y * w);
+  e[9] = 2.0f * (y * z - x * w);
+  e[10] = 1.0f - 2.0f * (x * x + y * y);
+  e[11] = 0.0f;
+  e[12] = e[13] = e[14] = 0.0f;
+  e[15] = 1.0f;
+  return m;
+}
+
+// ---------------------------------------------------------------------------------
+// Volume access with clamp-to-edge indices (Project/SingleScalarVolume.core.h:66-83)
+// ---------------------------------------------------------------------------------
+template <class R>
+struct VolView {
+  const R* d;
+  int W, H, D;
+  R at(int x, int y, int z) const {
+    x = std::max(std::min(x, W - 1), 0);
+    y = std::max(std::min(y, H - 1), 0);
+    z = std::max(std::min(z, D - 1), 0);
+    return d[(size_t)x + (size_t)W * ((size_t)y + (size_t)H * (size_t)z)];
+  }
+};
+
+// the reference's min/max macros: ((a)<(b)?(a):(b)), ((a)>(b)?(a):(b))
+template <class T>
+inline T mmin(T a, T b) { return (a < b) ? a : b; }
+template <class T>
+inline T mmax(T a, T b) { return (a > b) ? a : b; }
+
+// Trilinear sample, Project/SingleScalarVolume.core.h:99-202 (forward branch).
+template <class R>
+inline R trilinear(const VolView<R>& V, R p0, R p1, R p2) {
+  p0 = mmax(mmin(p0, (R)(V.W - 1)), (R)0);
+  p1 = mmax(mmin(p1, (R)(V.H - 1)), (R)0);
+  p2 = mmax(mmin(p2, (R)(V.D - 1)), (R)0);
+  const int i0 = (int)p0, i1 = (int)p1, i2 = (int)p2;
+  R f2 = p0 - (R)i0;
+  R f1 = (R)1 - f2;
+  R a = V.at(i0, i1, i2) * f1 + V.at(i0 + 1, i1, i2) * f2;
+  R b = V.at(i0, i1 + 1, i2) * f1 + V.at(i0 + 1, i1 + 1, i2) * f2;
+  R c = V.at(i0, i1, i2 + 1) * f1 + V.at(i0 + 1, i1, i2 + 1) * f2;
+  R d = V.at(i0, i1 + 1, i2 + 1) * f1 + V.at(i0 + 1, i1 + 1, i2 + 1) * f2;
+  f2 = p1 - (R)i1;
+  f1 = (R)1 - f2;
+  a = a * f1 + b * f2;
+  c = c * f1 + d * f2;
+  f2 = p2 - (R)i2;
+  f1 = (R)1 - f2;
+  return a * f1 + c * f2;
+}
+
+// Backprojection branch of the same routine (Project/SingleScalarVolume.core.h:85-96,
+// 115-121, 149-199): scatter corr*w and w to the 8 corners. Corners beyond the upper
+// face always carry weight exactly 0 (positions are clamped to [0, dim-1]) and are
+// skipped; the reference writes those zero contributions out of bounds.
+template <class R>
+inline void trilinear_scatter(int W, int H, int D, R p0, R p1, R p2, R cr, R scale, R* out,
+                              R* wsum) {
+  p0 = mmax(mmin(p0, (R)(W - 1)), (R)0);
+  p1 = mmax(mmin(p1, (R)(H - 1)), (R)0);
+  p2 = mmax(mmin(p2, (R)(D - 1)), (R)0);
+  const int i0 = (int)p0, i1 = (int)p1, i2 = (int)p2;
+  const R t0 = p0 - (R)i0, t1 = p1 - (R)i1, t2 = p2 - (R)i2;
+  const R m0 = (R)1 - t0, m1 = (R)1 - t1, m2 = (R)1 - t2;
+  const R w[8] = {m0 * m1 * m2, t0 * m1 * m2, m0 * t1 * m2, t0 * t1 * m2,
+                  m0 * m1 * t2, t0 * m1 * t2, m0 * t1 * t2, t0 * t1 * t2};
+  for (int k = 0; k < 8; ++k) {
+    const int x = i0 + (k & 1), y = i1 + ((k >> 1) & 1), z = i2 + ((k >> 2) & 1);
+    if (x >= W || y >= H || z >= D) continue;
+    const size_t idx = (size_t)x + (size_t)W * ((size_t)y + (size_t)H * (size_t)z);
+    out[idx] += scale * (cr * w[k]);
+    if (wsum) wsum[idx] += w[k];
+  }
+}
+
+// ---------------------------------------------------------------------------------
+// Ray setup and march
+// ---------------------------------------------------------------------------------
+template <class R>
+struct Ray {
+  R zero[4], unit[4];
+  R tnear, tfar;
+};
+
+template <class R>
+inline void matvec(const float* M, const R* v, R* out) {
+  for (int i = 0; i < 4; ++i)
+    out[i] = (R)M[i] * v[0] + (R)M[4 + i] * v[1] + (R)M[8 + i] * v[2] + (R)M[12 + i] * v[3];
+}
+
+// ProduceRay, Project/TypicalRayProducer.core.h:32-69, with the cropping bounds
+// [0, dim-1]^3 of GenerateImage_GetCorrections (:241-244). Pixel coordinates are the
+// integer indices (:195). Parallel beam: origin = P2M*(i,j,0,1), fixed direction.
+template <class R>
+inline Ray<R> produce_ray(const WapView& v, const int* dims, int i, int j) {
+  Ray<R> r;
+  R pix[4] = {(R)(float)i, (R)(float)j, (R)0, (R)1};
+  R mp[4];
+  matvec<R>(v.p2m, pix, mp);
+  R origin[4], dir[4];
+  if (!v.parallel) {
+    const R inv = (R)1 / mp[3];
+    for (int k = 0; k < 4; ++k) mp[k] *= inv;
+    for (int k = 0; k < 4; ++k) origin[k] = (R)v.eye[k];
+    for (int k = 0; k < 4; ++k) dir[k] = mp[k] - origin[k];
+    const R li = (R)1 / std::sqrt(dir[0] * dir[0] + dir[1] * dir[1] + dir[2] * dir[2]);
+    dir[0] *= li;
+    dir[1] *= li;
+    dir[2] *= li;
+  } else {
+    for (int k = 0; k < 4; ++k) origin[k] = mp[k];
+    for (int k = 0; k < 4; ++k) dir[k] = (R)v.dir[k];
+  }
+  matvec<R>(v.m2v, origin, r.zero);
+  matvec<R>(v.m2v, dir, r.unit);
+  const R cb[6] = {(R)0, (R)(float)(dims[0] - 1), (R)0, (R)(float)(dims[1] - 1), (R)0,
+                   (R)(float)(dims[2] - 1)};
+  R tmin[3], tmax[3];
+  for (int a = 0; a < 3; ++a) {
+    const R t1 = (cb[2 * a] - r.zero[a]) / r.unit[a];
+    const R t2 = (cb[2 * a + 1] - r.zero[a]) / r.unit[a];
+    tmin[a] = mmin(t1, t2);
+    tmax[a] = mmax(t1, t2);
+  }
+  R tn = mmax(mmax(tmin[0], tmin[1]), tmin[2]);
+  r.tnear = mmax(tn, (R)0);
+  r.tfar = mmin(mmin(tmax[0], tmax[1]), tmax[2]);
+  return r;
+}
+
+// CastingRay, Project/RayCaster.core.h:42-188, with the DDA "blocks" equal to the
+// whole volume (RayCasterPrepare(bsize = voldims), Project/WaP_Tomo.cpp:1750). Calls
+// visit(pos, R) for every sample; the tnear sample is visited twice (:60-61 and the
+// first inner iteration at t = tnear).
+template <class R, class F>
+inline void cast_ray(const Ray<R>& r, const int* dims, R sd, F&& visit) {
+  const R tnear = r.tnear, tfar = r.tfar;
+  R t1 = tnear;
+  R pos1[3];
+  for (int a = 0; a < 3; ++a) pos1[a] = r.zero[a] + r.unit[a] * t1;
+  R step[3];
+  for (int a = 0; a < 3; ++a) step[a] = r.unit[a] * sd;
+  visit(pos1);
+  R tMax[3], tDelta[3] = {(R)0, (R)0, (R)0};
+  for (int a = 0; a < 3; ++a) {
+    const int bs = dims[a];
+    if (r.unit[a] == (R)0) {
+      tMax[a] = tfar;
+    } else if (r.unit[a] < (R)0) {
+      const int x = ((int)std::ceil(pos1[a] - (R)0.5f) - 1) / bs;
+      tMax[a] = ((R)(float)(x * bs) + (R)0.5f - r.zero[a]) / r.unit[a];
+      tDelta[a] = (R)(float)(-bs) / r.unit[a];
+    } else {
+      const int x = (int)std::floor(pos1[a] - (R)0.5f) / bs;
+      tMax[a] = ((R)(float)((x + 1) * bs) + (R)0.5f - r.zero[a]) / r.unit[a];
+      tDelta[a] = (R)(float)bs / r.unit[a];
+    }
+  }
+  while (t1 < tfar) {
+    R tb = mmin(mmin(tMax[0], tMax[1]), tMax[2]);
+    if (tb > tfar) tb = tfar;
+    R t = std::ceil((t1 - tnear) / sd) * sd + tnear;
+    R pos[3];
+    for (int a = 0; a < 3; ++a) pos[a] = r.zero[a] + r.unit[a] * t;
+    while (t < tb) {
+      visit(pos);
+      t += sd;
+      for (int a = 0; a < 3; ++a) pos[a] += step[a];
+    }
+    if (tMax[0] < tMax[1]) {
+      if (tMax[0] < tMax[2])
+        tMax[0] += tDelta[0];
+      else
+        tMax[2] += tDelta[2];
+    } else {
+      if (tMax[1] < tMax[2])
+        tMax[1] += tDelta[1];
+      else
+        tMax[2] += tDelta[2];
+    }
+    t1 = tb;
+  }
+}
+
+template <class R>
+void forward_project(const R* vol, const int* dims, R sd, const WapView& v, int nu, int nv,
+                     R* proj, R* raylen, R* weight) {
+  VolView<R> V{vol, dims[0], dims[1], dims[2]};
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int j = 0; j < nv; ++j) {
+    for (int i = 0; i < nu; ++i) {
+      const int k = i + j * nu;
+      Ray<R> r = produce_ray<R>(v, dims, i, j);
+      R density = 0, sw = 0;
+      raylen[k] = 0;
+      if (r.tnear < r.tfar) {
+        raylen[k] = r.tfar - r.tnear;
+        cast_ray<R>(r, dims, sd, [&](const R* p) {
+          sw += (R)1;
+          density += trilinear(V, p[0], p[1], p[2]) * sd;
+        });
+      }
+      weight[k] = sw;
+      proj[k] = density;
+    }
+  }
+}
+
+template <class R>
+void backproject_ray(const R* corr, const int* dims, R sd, const WapView& v, int nu, int nv,
+                     R scale, R* out, R* wsum) {
+  for (int j = 0; j < nv; ++j) {
+    for (int i = 0; i < nu; ++i) {
+      const int k = i + j * nu;
+      Ray<R> r = produce_ray<R>(v, dims, i, j);
+      if (!(r.tnear < r.tfar)) continue;
+      const R cr = corr[k];
+      cast_ray<R>(r, dims, sd, [&](const R* p) {
+        trilinear_scatter<R>(dims[0], dims[1], dims[2], p[0], p[1], p[2], cr, scale, out, wsum);
+      });
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------------
+// Voxel-driven bicubic backprojector, Project/WaP_Tomo.cpp:208-222 (cubic/bicubic in
+// double), :1557-1628 (prox update), :1294-1374 (delta accumulation)
+// ---------------------------------------------------------------------------------
+template <class R>
+inline R cubic_bp(const R* p, R x) {
+  // p[1] + 0.5*x*(p[2]-p[0] + x*(2p0 - 5p1 + 4p2 - p3 + x*(3(p1-p2) + p3 - p0))):
+  // p[2]-p[0] and p[1]-p[2] are float subtractions, the rest runs in double.
+  const R d20 = p[2] - p[0];
+  const R d12 = p[1] - p[2];
+  const double inner =
+      (double)d20 + (double)x * (2.0 * p[0] - 5.0 * p[1] + 4.0 * p[2] - p[3] +
+                                 (double)x * (3.0 * (double)d12 + p[3] - p[0]));
+  return (R)(p[1] + 0.5 * (double)x * inner);
+}
+
+template <class R>
+inline R bicubic_bp(const R p[4][4], R x, R y) {
+  R arr[4];
+  for (int i = 0; i < 4; ++i) arr[i] = cubic_bp<R>(p[i], y);
+  return cubic_bp<R>(arr, x);
+}
+
+template <class R>
+void backproject_voxel(R* vol, const int* dims, const R* corr, const WapView& v, int nu, int nv,
+                       double alpha, R gmax, const int* lo, const int* hi, int mode) {
+  M4 A;
+  std::memcpy(A.e, v.mvv2m, sizeof(A.e));
+#pragma omp parallel for schedule(static)
+  for (int z = lo[2]; z < hi[2]; ++z) {
+    for (int y = lo[1]; y < hi[1]; ++y) {
+      for (int x = lo[0]; x < hi[0]; ++x) {
+        R vec[4] = {(R)(float)x, (R)(float)y, (R)(float)z, (R)1};
+        R det[4];
+        matvec<R>(A.e, vec, det);
+        R fpx, fpy;
+        if (!v.parallel) {
+          fpx = (R)(v.half_w - (double)((det[0] / det[2]) * (R)v.sdd));
+          fpy = (R)(v.half_h - (double)((det[1] / det[2]) * (R)v.sdd));
+        } else {
+          fpx = (R)(v.half_w + (double)det[0]);
+          fpy = (R)(v.half_h + (double)det[1]);
+        }
+        fpx = (R)((double)fpx / v.ds);
+        fpy = (R)((double)fpy / v.ds);
+        const int px = (int)fpx, py = (int)fpy;
+        const R dx = fpx - (R)px, dy = fpy - (R)py;
+        if (px < 2 || px >= nu - 2 || py < 2 || py >= nv - 2) continue;
+        const R* pc = corr + (size_t)py * nu + px;
+        R pim[4][4];
+        for (int a = 0; a < 4; ++a)
+          for (int b = 0; b < 4; ++b) pim[a][b] = pc[(a - 1) + (b - 1) * nu];
+        const R g = bicubic_bp<R>(pim, dx, dy);
+        const size_t idx = (size_t)x + (size_t)dims[0] * ((size_t)y + (size_t)dims[1] * z);
+        if (mode == 0) {
+          R val = (R)((double)vol[idx] + alpha * (double)g);
+          val = mmin(gmax, val);
+          vol[idx] = mmax(val, (R)0);
+        } else {
+          vol[idx] += g;
+        }
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------------
+// Advection, Project/advection.h:10-146, CImg _cubic_atXYZ Project/CImg.h:13429-13503
+// ---------------------------------------------------------------------------------
+template <class R>
+inline R cimg_cubic1(R Ip, R Ic, R In, R Ia, R d) {
+  return Ic + (R)0.5f * (d * (-Ip + In) + d * d * ((R)2 * Ip - (R)5 * Ic + (R)4 * In - Ia) +
+                         d * d * d * (-Ip + (R)3 * Ic - (R)3 * In + Ia));
+}
+
+template <class R>
+inline R cimg_cubic_xyz(const R* img, int W, int H, int D, R fx, R fy, R fz) {
+  auto cutv = [](R v, R lo, R hi) { return v < lo ? lo : (v > hi ? hi : v); };
+  const R nfx = cutv(fx, (R)0, (R)(W - 1)), nfy = cutv(fy, (R)0, (R)(H - 1)),
+          nfz = cutv(fz, (R)0, (R)(D - 1));
+  const int x = (int)nfx, y = (int)nfy, z = (int)nfz;
+  const R dx = nfx - (R)x, dy = nfy - (R)y, dz = nfz - (R)z;
+  const int xs[4] = {x - 1 < 0 ? 0 : x - 1, x, dx > 0 ? x + 1 : x, x + 2 >= W ? W - 1 : x + 2};
+  const int ys[4] = {y - 1 < 0 ? 0 : y - 1, y, dy > 0 ? y + 1 : y, y + 2 >= H ? H - 1 : y + 2};
+  const int zs[4] = {z - 1 < 0 ? 0 : z - 1, z, dz > 0 ? z + 1 : z, z + 2 >= D ? D - 1 : z + 2};
+  R Iz[4];
+  for (int c = 0; c < 4; ++c) {
+    R Iy[4];
+    for (int b = 0; b < 4; ++b) {
+      const R* row = img + (size_t)W * ((size_t)ys[b] + (size_t)H * zs[c]);
+      Iy[b] = cimg_cubic1<R>(row[xs[0]], row[xs[1]], row[xs[2]], row[xs[3]], dx);
+    }
+    Iz[c] = cimg_cubic1<R>(Iy[0], Iy[1], Iy[2], Iy[3], dy);
+  }
+  return cimg_cubic1<R>(Iz[0], Iz[1], Iz[2], Iz[3], dz);
+}
+
+template <class R>
+struct Adv {
+  int dim[3];
+  R aabb[6];
+  // world_to_cell_grid (:32-36): float expression, then "- 0.5" in double
+  inline void w2c(const R* w, R* g) const {
+    for (int a = 0; a < 3; ++a)
+      g[a] = (R)((double)((R)dim[a] * (w[a] - aabb[2 * a]) / (aabb[2 * a + 1] - aabb[2 * a])) -
+                 0.5);
+  }
+  // cell_grid_to_world (:25-29): (grid + 0.5) in double
+  inline void c2w(int i, int j, int k, R* w) const {
+    const int g[3] = {i, j, k};
+    for (int a = 0; a < 3; ++a)
+      w[a] = (R)((double)aabb[2 * a] + ((double)(R)(float)g[a] + 0.5) *
+                                           (double)(aabb[2 * a + 1] - aabb[2 * a]) /
+                                           (double)(R)dim[a]);
+  }
+  // sample_image (:49-88): clamp to [1, dim-2], Catmull-Rom, clamp to the 8-cell range
+  inline R sample(const R* img, const R* p) const {
+    R g[3];
+    w2c(p, g);
+    for (int a = 0; a < 3; ++a) {
+      const R hi = (R)(dim[a] - 2);
+      const R m = (g[a] < hi) ? g[a] : hi;  // std::min(hi, g)
+      g[a] = ((R)1 < m) ? m : (R)1;         // std::max(1, m)
+    }
+    const int ix = (int)g[0], iy = (int)g[1], iz = (int)g[2];
+    const int W = dim[0], H = dim[1], D = dim[2];
+    auto at = [&](int x, int y, int z) {
+      return img[(size_t)x + (size_t)W * ((size_t)y + (size_t)H * z)];
+    };
+    R mn = at(ix, iy, iz), mx = mn;
+    const int o[7][3] = {{1, 0, 0}, {1, 1, 0}, {0, 1, 0}, {0, 0, 1},
+                         {1, 0, 1}, {1, 1, 1}, {0, 1, 1}};
+    for (int q = 0; q < 7; ++q) {
+      const R s = at(ix + o[q][0], iy + o[q][1], iz + o[q][2]);
+      mn = (s < mn) ? s : mn;  // std::min(mn, s)
+      mx = (mx < s) ? s : mx;  // std::max(mx, s)
+    }
+    const R c = cimg_cubic_xyz<R>(img, W, H, D, g[0], g[1], g[2]);
+    const R lo = (c < mx) ? c : mx;  // std::min(mx, c)
+    return (mn < lo) ? lo : mn;      // std::max(mn, lo)
+  }
+  inline void vel(const R* v, const R* p, R* out) const {
+    const size_t n = (size_t)dim[0] * dim[1] * dim[2];
+    for (int c = 0; c < 3; ++c) out[c] = sample(v + c * n, p);
+  }
+  // trace (:101-115): p2 = p + dt*v(p); p0 = p2 - dt*v(p2); p2 + 0.5*(p - p0)
+  inline void trace(const R* v, const R* p, R dt, R* q) const {
+    R s[3], p2[3], p0[3];
+    vel(v, p, s);
+    for (int a = 0; a < 3; ++a) p2[a] = p[a] + dt * s[a];
+    vel(v, p2, s);
+    for (int a = 0; a < 3; ++a) p0[a] = p2[a] - dt * s[a];
+    for (int a = 0; a < 3; ++a) q[a] = p2[a] + (R)(0.5 * (double)(p[a] - p0[a]));
+  }
+};
+
+template <class R>
+void advect(const int* dims, const R* aabb, const R* vel, const R* src, int nc, R dt, R* out) {
+  Adv<R> A;
+  for (int a = 0; a < 3; ++a) A.dim[a] = dims[a];
+  for (int a = 0; a < 6; ++a) A.aabb[a] = aabb[a];
+  const size_t n = (size_t)dims[0] * dims[1] * dims[2];
+#pragma omp parallel for schedule(static)
+  for (int k = 0; k < dims[2]; ++k) {
+    for (int j = 0; j < dims[1]; ++j) {
+      for (int i = 0; i < dims[0]; ++i) {
+        R p[3], q[3];
+        A.c2w(i, j, k, p);
+        A.trace(vel, p, -dt, q);
+        const size_t idx = (size_t)i + (size_t)dims[0] * ((size_t)j + (size_t)dims[1] * k);
+        for (int c = 0; c < nc; ++c) out[idx + c * n] = A.sample(src + c * n, q);
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------------
+// CImg image ops used by the flow solver
+// ---------------------------------------------------------------------------------
+// get_gradient("xyz", 0): central differences with Neumann borders, Project/CImg.h:34570-34582
+template <class R>
+void gradient(const R* s, const int* dims, R* out3) {
+  const int W = dims[0], H = dims[1], D = dims[2];
+  const size_t n = (size_t)W * H * D;
+#pragma omp parallel for schedule(static)
+  for (int z = 0; z < D; ++z)
+    for (int y = 0; y < H; ++y)
+      for (int x = 0; x < W; ++x) {
+        auto at = [&](int a, int b, int c) {
+          a = std::max(0, std::min(W - 1, a));
+          b = std::max(0, std::min(H - 1, b));
+          c = std::max(0, std::min(D - 1, c));
+          return s[(size_t)a + (size_t)W * ((size_t)b + (size_t)H * c)];
+        };
+        const size_t i = (size_t)x + (size_t)W * ((size_t)y + (size_t)H * z);
+        out3[i] = (at(x + 1, y, z) - at(x - 1, y, z)) / (R)2;
+        out3[n + i] = (at(x, y + 1, z) - at(x, y - 1, z)) / (R)2;
+        out3[2 * n + i] = (at(x, y, z + 1) - at(x, y, z - 1)) / (R)2;
+      }
+}
+
+// Deriche order-0 recursive smoothing along one axis with Neumann start values,
+// Project/CImg.h:32917-33008. Coefficients are computed in float (exp of a float).
+template <class R>
+struct DericheCoef {
+  R a0, a1, a2, a3, b1, b2, coefp, coefn;
+};
+
+template <class R>
+DericheCoef<R> deriche_coef(R sigma) {
+  DericheCoef<R> c;
+  const R ns = sigma < (R)0.1f ? (R)0.1f : sigma;
+  const R alpha = (R)1.695f / ns;
+  const R ema = std::exp(-alpha), ema2 = std::exp(-2 * alpha);
+  c.b1 = -2 * ema;
+  c.b2 = ema2;
+  const R k = ((R)1 - ema) * ((R)1 - ema) / ((R)1 + 2 * alpha * ema - ema2);
+  c.a0 = k;
+  c.a1 = k * (alpha - (R)1) * ema;
+  c.a2 = k * (alpha + (R)1) * ema;
+  c.a3 = -k * ema2;
+  c.coefp = (c.a0 + c.a1) / ((R)1 + c.b1 + c.b2);
+  c.coefn = (c.a2 + c.a3) / ((R)1 + c.b1 + c.b2);
+  return c;
+}
+
+template <class R>
+void deriche_line(R* x, int N, size_t off, const DericheCoef<R>& c, R* Y) {
+  R yb = 0, yp = 0, xp = 0;
+  xp = x[0];
+  yb = yp = c.coefp * xp;
+  for (int m = 0; m < N; ++m) {
+    const R xc = x[(size_t)m * off];
+    const R yc = Y[m] = c.a0 * xc + c.a1 * xp - c.b1 * yp - c.b2 * yb;
+    xp = xc;
+    yb = yp;
+    yp = yc;
+  }
+  R xn = x[(size_t)(N - 1) * off], xa = xn;
+  R yn = c.coefn * xn, ya = yn;
+  for (int n = N - 1; n >= 0; --n) {
+    const R xc = x[(size_t)n * off];
+    const R yc = c.a2 * xn + c.a3 * xa - c.b1 * yn - c.b2 * ya;
+    xa = xn;
+    xn = xc;
+    ya = yn;
+    yn = yc;
+    x[(size_t)n * off] = Y[n] + yc;
+  }
+}
+
+template <class R>
+void blur(const R* src, const int* dims, int nc, R sigma, R* out) {
+  const int W = dims[0], H = dims[1], D = dims[2];
+  const size_t n = (size_t)W * H * D;
+  std::copy(src, src + n * nc, out);
+  if (sigma < (R)0.1f) return;  // nsigma < 0.1 and order 0: unchanged
+  const DericheCoef<R> c = deriche_coef<R>(sigma);
+  for (int ch = 0; ch < nc; ++ch) {
+    R* img = out + ch * n;
+    if (W > 1) {
+#pragma omp parallel for schedule(static)
+      for (int zy = 0; zy < H * D; ++zy) {
+        std::vector<R> Y(W);
+        deriche_line<R>(img + (size_t)zy * W, W, 1, c, Y.data());
+      }
+    }
+    if (H > 1) {
+#pragma omp parallel for schedule(static)
+      for (int zx = 0; zx < W * D; ++zx) {
+        const int x = zx % W, z = zx / W;
+        std::vector<R> Y(H);
+        deriche_line<R>(img + x + (size_t)W * H * z, H, (size_t)W, c, Y.data());
+      }
+    }
+    if (D > 1) {
+#pragma omp parallel for schedule(static)
+      for (int yx = 0; yx < W * H; ++yx) {
+        std::vector<R> Y(D);
+        deriche_line<R>(img + yx, D, (size_t)W * H, c, Y.data());
+      }
+    }
+  }
+}
+
+// One axis of get_resize(..., interpolation 5): moving average when shrinking (case 2,
+// Project/CImg.h:28137-28212), cubic with offset tables when growing (case 5,
+// :28434-28520), nearest when the source axis has length 1 (case 1).
+// line(i) addresses element i of a line; the axis has length n -> s.
+template <class R>
+void resize_line(const R* src, size_t soff, int n, R* dst, size_t doff, int s) {
+  if (s == n) {
+    for (int i = 0; i < n; ++i) dst[(size_t)i * doff] = src[(size_t)i * soff];
+    return;
+  }
+  if (n == 1) {
+    for (int i = 0; i < s; ++i) dst[(size_t)i * doff] = src[0];
+    return;
+  }
+  if (n > s) {
+    std::vector<R> tmp(s, (R)0);
+    unsigned a = (unsigned)n * s, b = n, c = s, si = 0, t = 0;
+    while (a) {
+      const unsigned d = std::min(b, c);
+      a -= d;
+      b -= d;
+      c -= d;
+      tmp[t] += src[(size_t)si * soff] * (R)d;
+      if (!b) {
+        tmp[t] /= (R)n;
+        ++t;
+        b = n;
+      }
+      if (!c) {
+        ++si;
+        c = s;
+      }
+    }
+    for (int i = 0; i < s; ++i) dst[(size_t)i * doff] = tmp[i];
+    return;
+  }
+  const double f = s > 1 ? (n - 1.0) / (s - 1) : 0;
+  double curr = 0, old = 0;
+  long p = 0;
+  for (int x = 0; x < s; ++x) {
+    const double t = curr - (unsigned)curr;
+    old = curr;
+    curr = std::min(n - 1.0, curr + f);
+    const double v1 = (double)src[(size_t)p * soff];
+    const double v0 = p > 0 ? (double)src[(size_t)(p - 1) * soff] : v1;
+    const double v2 = p <= n - 2 ? (double)src[(size_t)(p + 1) * soff] : v1;
+    const double v3 = p < n - 2 ? (double)src[(size_t)(p + 2) * soff] : v2;
+    const double val = v1 + 0.5f * (t * (-v0 + v2) + t * t * (2 * v0 - 5 * v1 + 4 * v2 - v3) +
+                                    t * t * t * (-v0 + 3 * v1 - 3 * v2 + v3));
+    const double lo = -(double)3.4028234663852886e38, hi = (double)3.4028234663852886e38;
+    dst[(size_t)x * doff] = (R)(val < lo ? lo : (val > hi ? hi : val));
+    p += (long)((unsigned)curr - (unsigned)old);
+  }
+}
+
+template <class R>
+void resize(const R* src, const int* dims, int nc, const int* nd, R* out) {
+  const int W = dims[0], H = dims[1], D = dims[2];
+  const int sx = nd[0], sy = nd[1], sz = nd[2];
+  std::vector<R> ax((size_t)sx * H * D * nc), ay((size_t)sx * sy * D * nc);
+  for (int c = 0; c < nc; ++c) {
+    const R* s = src + (size_t)c * W * H * D;
+    R* a = ax.data() + (size_t)c * sx * H * D;
+    for (int zy = 0; zy < H * D; ++zy) resize_line<R>(s + (size_t)zy * W, 1, W, a + (size_t)zy * sx, 1, sx);
+    R* b = ay.data() + (size_t)c * sx * sy * D;
+    for (int z = 0; z < D; ++z)
+      for (int x = 0; x < sx; ++x)
+        resize_line<R>(a + x + (size_t)sx * H * z, sx, H, b + x + (size_t)sx * sy * z, sx, sy);
+    R* o = out + (size_t)c * sx * sy * sz;
+    for (int yx = 0; yx < sx * sy; ++yx)
+      resize_line<R>(b + yx, (size_t)sx * sy, D, o + yx, (size_t)sx * sy, sz);
+  }
+}
+
+// ---------------------------------------------------------------------------------
+// L1 optical-flow primal-dual solver, Project/optical_flow.h:87-995; finite
+// differences Project/tools.cpp:247-314
+// ---------------------------------------------------------------------------------
+struct Grid {
+  int W, H, D;
+  size_t idx(int i, int j, int k) const {
+    return (size_t)i + (size_t)W * ((size_t)j + (size_t)H * k);
+  }
+};
+
+template <class R>
+inline R dp(const R* a, const Grid& g, int i, int j, int k, int axis) {
+  switch (axis) {
+    case 0: return i < g.W - 1 ? a[g.idx(i + 1, j, k)] - a[g.idx(i, j, k)] : (R)0;
+    case 1: return j < g.H - 1 ? a[g.idx(i, j + 1, k)] - a[g.idx(i, j, k)] : (R)0;
+    default: return k < g.D - 1 ? a[g.idx(i, j, k + 1)] - a[g.idx(i, j, k)] : (R)0;
+  }
+}
+
+template <class R>
+inline R dm(const R* a, const Grid& g, int i, int j, int k, int axis) {
+  const int c = axis == 0 ? i : (axis == 1 ? j : k);
+  const int n = axis == 0 ? g.W : (axis == 1 ? g.H : g.D);
+  const size_t here = g.idx(i, j, k);
+  const size_t prev = axis == 0 ? g.idx(i - 1, j, k)
+                                : (axis == 1 ? g.idx(i, j - 1, k) : g.idx(i, j, k - 1));
+  if (c > 0 && c < n - 1) return a[here] - a[prev];
+  if (c == 0) return a[here];
+  return -a[prev];
+}
+
+template <class R>
+void flow_single(const int* dims, const R* I1, const R* I2, R* V, R* Y1, R* Y2, R* Y3, R* Y4,
+                 R tol, R lambda, int max_iter, int nwarps, const R* aabb, const int* roi,
+                 int* iters_run) {
+  const Grid g{dims[0], dims[1], dims[2]};
+  const size_t n = (size_t)g.W * g.H * g.D;
+  const int nPx = (int)(dims[0] * dims[1] * dims[2]);
+  const R sigma1 = std::min((R)(1.0f / 3.0f), std::min((R)(1.0f / 3.0f), (R)(1.0f / 3.0f)));
+  std::vector<R> v[3], vOld[3], y[10], yOld[10], Kty[3], KtyOld[3], Kx[10], KxOld[10];
+  std::vector<R> u[3], ut(n, (R)0), tau[3], sig2(n);
+  for (int c = 0; c < 3; ++c) {
+    v[c].assign(V + c * n, V + (c + 1) * n);
+    vOld[c].assign(n, (R)0);
+    Kty[c].assign(n, (R)0);
+    KtyOld[c].assign(n, (R)0);
+    u[c].assign(n, (R)0);
+    tau[c].assign(n, (R)0);
+  }
+  R* Yc[3] = {Y1, Y2, Y3};
+  for (int c = 0; c < 3; ++c)
+    for (int d = 0; d < 3; ++d) y[3 * c + d].assign(Yc[c] + d * n, Yc[c] + (d + 1) * n);
+  y[9].assign(Y4, Y4 + n);
+  for (int q = 0; q < 10; ++q) {
+    yOld[q].assign(n, (R)0);
+    Kx[q].assign(n, (R)0);
+    KxOld[q].assign(n, (R)0);
+  }
+  const int i0 = roi[0], i1 = g.W - roi[1], j0 = roi[2], j1 = g.H - roi[3], k0 = roi[4],
+            k1 = g.D - roi[5];
+  std::vector<R> warped(n), grad(3 * n), pterm(n), dterm(n);
+  int total = 0;
+  for (int w = 0; w < nwarps; ++w) {
+    advect<R>(dims, aabb, V, I2, 1, (R)-1, warped.data());
+    gradient<R>(warped.data(), dims, grad.data());
+#pragma omp parallel for schedule(static)
+    for (int k = k0; k < k1; ++k)
+      for (int j = j0; j < j1; ++j)
+        for (int i = i0; i < i1; ++i) {
+          const size_t t = g.idx(i, j, k);
+          for (int c = 0; c < 3; ++c) u[c][t] = grad[c * n + t];
+          ut[t] = warped[t] - I1[t] - u[0][t] * v[0][t] - u[1][t] * v[1][t] - u[2][t] * v[2][t];
+        }
+    for (size_t t = 0; t < n; ++t) {
+      for (int c = 0; c < 3; ++c) {
+        const R a = u[c][t] > 0 ? u[c][t] : -u[c][t];  // myAbs, tools.cpp:76-79
+        tau[c][t] = (R)1 / ((R)4.0f / (R)1.0f + a);
+      }
+      sig2[t] = (R)1 / (std::abs(u[0][t]) + std::abs(u[1][t]) + std::abs(u[2][t]));
+    }
+    int it = 0;
+    R err = (R)1;
+    R p = 0, d = 0;
+    while (err > tol && it <= max_iter) {
+      ++it;
+      const bool res = (it % 50 == 0);
+      if (res) p = d = 0;
+      // primal over the ROI
+#pragma omp parallel for schedule(static)
+      for (int k = k0; k < k1; ++k)
+        for (int j = j0; j < j1; ++j)
+          for (int i = i0; i < i1; ++i) {
+            const size_t t = g.idx(i, j, k);
+            R pt = 0;
+            for (int c = 0; c < 3; ++c) {
+              if (res) {
+                vOld[c][t] = v[c][t];
+                KtyOld[c][t] = Kty[c][t];
+              }
+              Kty[c][t] = -(R)1.0f * dm(y[3 * c].data(), g, i, j, k, 0) -
+                          (R)1.0f * dm(y[3 * c + 1].data(), g, i, j, k, 1) -
+                          (R)1.0f * dm(y[3 * c + 2].data(), g, i, j, k, 2) + u[c][t] * y[9][t];
+              v[c][t] -= tau[c][t] * Kty[c][t];
+              V[c * n + t] = v[c][t];
+            }
+            if (res) {
+              for (int c = 0; c < 3; ++c)
+                pt += std::abs((vOld[c][t] - v[c][t]) / tau[c][t] - KtyOld[c][t] + Kty[c][t]);
+              pterm[t] = pt;
+            }
+          }
+      if (res) {
+        for (int k = k0; k < k1; ++k)
+          for (int j = j0; j < j1; ++j)
+            for (int i = i0; i < i1; ++i) p += pterm[g.idx(i, j, k)];
+      }
+      // dual over every voxel
+#pragma omp parallel for schedule(static)
+      for (int k = 0; k < g.D; ++k)
+        for (int j = 0; j < g.H; ++j)
+          for (int i = 0; i < g.W; ++i) {
+            const size_t t = g.idx(i, j, k);
+            if (res)
+              for (int q = 0; q < 10; ++q) yOld[q][t] = y[q][t];
+            for (int q = 0; q < 10; ++q) KxOld[q][t] = Kx[q][t];
+            for (int c = 0; c < 3; ++c)
+              for (int a = 0; a < 3; ++a) Kx[3 * c + a][t] = (R)1.0f * dp(v[c].data(), g, i, j, k, a);
+            Kx[9][t] = u[0][t] * v[0][t] + u[1][t] * v[1][t] + u[2][t] * v[2][t];
+            for (int q = 0; q < 9; ++q) {
+              const R yt = y[q][t] + sigma1 * (Kx[q][t] + Kx[q][t] - KxOld[q][t]);
+              y[q][t] = yt * lambda / (lambda + 2 * sigma1);
+            }
+            const R y4n = y[9][t] + sig2[t] * (Kx[9][t] + Kx[9][t] - KxOld[9][t] + ut[t]);
+            const R m1 = (y4n < (R)1.0f) ? y4n : (R)1.0f;   // std::min(1.0f, y4n)
+            y[9][t] = ((R)-1.0f < m1) ? m1 : (R)-1.0f;      // std::max(-1.0f, m1)
+            if (res) {
+              R dt = 0;
+              for (int q = 0; q < 9; ++q)
+                dt += std::abs((yOld[q][t] - y[q][t]) / sigma1 - KxOld[q][t] + Kx[q][t]);
+              dt += std::abs((yOld[9][t] - y[9][t]) / sig2[t] - KxOld[9][t] + Kx[9][t]);
+              dterm[t] = dt;
+            }
+          }
+      if (res) {
+        for (size_t t = 0; t < n; ++t) d += dterm[t];
+        err = (d * d + p * p) / (R)nPx;
+      }
+    }
+    total += it;
+  }
+  // write back over the ROI only (:836-867)
+  for (int k = k0; k < k1; ++k)
+    for (int j = j0; j < j1; ++j)
+      for (int i = i0; i < i1; ++i) {
+        const size_t t = g.idx(i, j, k);
+        for (int c = 0; c < 3; ++c)
+          for (int d2 = 0; d2 < 3; ++d2) Yc[c][d2 * n + t] = y[3 * c + d2][t];
+        Y4[t] = y[9][t];
+        for (int c = 0; c < 3; ++c) V[c * n + t] = v[c][t];
+      }
+  if (iters_run) *iters_run = total;
+}
+
+// L1TVOpticalFlowNonlinear_MultiScale3D, Project/optical_flow.h:1002-1126
+template <class R>
+void flow_multiscale(const int* dims, const R* I1, const R* I2, const R* V_init, R tol,
+                     R lambda, int max_iter, int nwarps, R eta, R sigma, int scales,
+                     const int* cut, R* V_out) {
+  const size_t n0 = (size_t)dims[0] * dims[1] * dims[2];
+  std::vector<R> b1(n0), b2(n0);
+  blur<R>(I1, dims, 1, sigma, b1.data());
+  blur<R>(I2, dims, 1, sigma, b2.data());
+  struct Level {
+    int d[3];
+    std::vector<R> i1, i2, v, y1, y2, y3, y4;
+  };
+  std::vector<Level> lv(scales);
+  R t_eta = (R)1.0f;
+  for (int s = 0; s < scales; ++s) {
+    Level& L = lv[scales - 1 - s];  // reversed: index 0 is the coarsest
+    for (int a = 0; a < 3; ++a) L.d[a] = (int)((R)dims[a] * t_eta);
+    const size_t nl = (size_t)L.d[0] * L.d[1] * L.d[2];
+    L.i1.resize(nl);
+    L.i2.resize(nl);
+    resize<R>(b1.data(), dims, 1, L.d, L.i1.data());
+    resize<R>(b2.data(), dims, 1, L.d, L.i2.data());
+    L.v.assign(3 * nl, (R)0);
+    if (V_init) resize<R>(V_init, dims, 3, L.d, L.v.data());
+    L.y1.assign(3 * nl, (R)0);
+    L.y2.assign(3 * nl, (R)0);
+    L.y3.assign(3 * nl, (R)0);
+    L.y4.assign(nl, (R)0);
+    t_eta = t_eta * eta;
+  }
+  for (int s = 0; s < scales; ++s) {
+    Level& L = lv[s];
+    const R aabb[6] = {(R)0, (R)dims[0], (R)0, (R)dims[1], (R)0, (R)dims[2]};
+    int roi[6];
+    for (int q = 0; q < 6; ++q) {
+      const int a = q / 2;
+      roi[q] = (int)((float)cut[q] * ((float)L.d[a] / (float)dims[a]));
+    }
+    flow_single<R>(L.d, L.i1.data(), L.i2.data(), L.v.data(), L.y1.data(), L.y2.data(),
+                   L.y3.data(), L.y4.data(), tol, lambda, max_iter, nwarps, aabb, roi, nullptr);
+    if (s < scales - 1) {
+      Level& N = lv[s + 1];
+      resize<R>(L.v.data(), L.d, 3, N.d, N.v.data());
+      resize<R>(L.y1.data(), L.d, 3, N.d, N.y1.data());
+      resize<R>(L.y2.data(), L.d, 3, N.d, N.y2.data());
+      resize<R>(L.y3.data(), L.d, 3, N.d, N.y3.data());
+      resize<R>(L.y4.data(), L.d, 1, N.d, N.y4.data());
+    }
+  }
+  const Level& F = lv[scales - 1];
+  std::copy(F.v.begin(), F.v.end(), V_out);
+}
+
+// ---------------------------------------------------------------------------------
+// PSART loop pieces, Project/WaP_Tomo.cpp
+// ---------------------------------------------------------------------------------
+// SART correction + dual, :1530-1555 (Factor = sqrtf(2*SART_lambda) = 1 exactly)
+template <class R>
+void sart_correct(const R* b, const R* ax, const R* raylen, R* py, double alpha, R* corr,
+                  size_t n) {
+  for (size_t k = 0; k < n; ++k) {
+    corr[k] = (b[k] - ax[k] - py[k]) / (raylen[k] + (R)1.0f);
+    py[k] = (R)((double)py[k] + alpha * (double)corr[k]);
+  }
+}
+
+template <class R>
+void roi_apply(R* vol, const R* mask, size_t n) {
+  for (size_t k = 0; k < n; ++k) {
+    const R m = vol[k] * mask[k];
+    vol[k] = mmax(m, (R)0);
+  }
+}
+
+// SART_Proximal_Operator for one state, :1424-1678 (the dual is per view and zeroed
+// per call, which is the intended form of the mis-sized pY)
+template <class R>
+void sart_prox(const OrcLoop& L, const WapView* views, const R* sino, int first, int last,
+               R* vol, const R* mask) {
+  const size_t nd = (size_t)L.nu * L.nv;
+  const int nviews = last - first + 1;
+  std::vector<R> py((size_t)nviews * nd, (R)0), proj(nd), len(nd), wt(nd), corr(nd);
+  std::vector<int> order(nviews);
+  orc_view_order(first, last, order.data());
+  const int lo[3] = {L.roi[0], L.roi[2], L.roi[4]};
+  const int hi[3] = {L.dims[0] - L.roi[1], L.dims[1] - L.roi[3], L.dims[2] - L.roi[5]};
+  for (int s = 0; s < L.nb_sart; ++s) {
+    for (int q = 0; q < nviews; ++q) {
+      const int view = order[q];
+      forward_project<R>(vol, L.dims, (R)L.sd, views[view], L.nu, L.nv, proj.data(),
+                         len.data(), wt.data());
+      sart_correct<R>(sino + (size_t)view * nd, proj.data(), len.data(),
+                      py.data() + (size_t)(view - first) * nd, L.alpha, corr.data(), nd);
+      backproject_voxel<R>(vol, L.dims, corr.data(), views[view], L.nu, L.nv, L.alpha,
+                           (R)L.gmax, lo, hi, 0);
+    }
+  }
+  roi_apply<R>(vol, mask, (size_t)L.dims[0] * L.dims[1] * L.dims[2]);
+}
+
+// WarpAndProjct for one view, :1207-1421 (+ SingleScalarVolumePrepare's warp choice,
+// Project/SingleScalarVolume.core.h:392-404)
+template <class R>
+void warp_and_project(const OrcLoop& L, const WapView& view, const R* b, int frame, float rtime,
+                      R* vols, const R* flows, const R* mask) {
+  const size_t n = (size_t)L.dims[0] * L.dims[1] * L.dims[2];
+  const size_t nd = (size_t)L.nu * L.nv;
+  const int nf = L.nstates;
+  const R aabb[6] = {(R)0, (R)L.dims[0], (R)0, (R)L.dims[1], (R)0, (R)L.dims[2]};
+  R* vol = vols + (size_t)frame * n;
+  std::vector<R> warped(n), proj(nd), len(nd), wt(nd), corr(nd), py(nd, (R)0), delta(n, (R)0),
+      back(n);
+  const int fsel = rtime < 0.0f ? std::max(0, frame - 1) : std::min(nf - 1, frame);
+  advect<R>(L.dims, aabb, flows + (size_t)fsel * 3 * n, vol, 1, (R)rtime, warped.data());
+  forward_project<R>(warped.data(), L.dims, (R)L.sd, view, L.nu, L.nv, proj.data(), len.data(),
+                     wt.data());
+  sart_correct<R>(b, proj.data(), len.data(), py.data(), L.alpha, corr.data(), nd);
+  const int lo[3] = {0, 0, 0};
+  backproject_voxel<R>(delta.data(), L.dims, corr.data(), view, L.nu, L.nv, L.alpha, (R)0, lo,
+                       L.dims, 1);
+  advect<R>(L.dims, aabb, flows + (size_t)std::max(0, frame - 1) * 3 * n, delta.data(), 1,
+            (R)-rtime, back.data());
+  for (size_t k = 0; k < n; ++k) {
+    // fVolume += m_alpha * advect(...): CImg scales in double and stores float first
+    const R s = (R)(L.alpha * (double)back[k]);
+    R x = vol[k] + s;
+    x = x < (R)0.0 ? (R)0.0 : (x > (R)L.gmax ? (R)L.gmax : x);  // cut(0, G_Max)
+    vol[k] = x;
+  }
+  roi_apply<R>(vol, mask, n);
+}
+
+template <class R>
+void wap_iteration(const OrcLoop& L, const WapView* views, const R* sino, const int* ws,
+                   const int* we, const float* ftime, R* vols, R* flows, const R* mask) {
+  const size_t n = (size_t)L.dims[0] * L.dims[1] * L.dims[2];
+  const size_t nd = (size_t)L.nu * L.nv;
+  for (int f = 0; f < L.nstates; ++f)
+    sart_prox<R>(L, views, sino, ws[f], we[f], vols + (size_t)f * n, mask);
+  if (L.do_flow) {
+    for (int f = 0; f + 1 < L.nstates; ++f) {
+      std::vector<R> out(3 * n);
+      flow_multiscale<R>(L.dims, vols + (size_t)f * n, vols + (size_t)(f + 1) * n,
+                         flows + (size_t)f * 3 * n, (R)L.flow_tol, (R)L.flow_lambda,
+                         L.flow_iters, L.flow_warps, (R)L.flow_eta, (R)L.flow_sigma,
+                         L.flow_scales, L.cut, out.data());
+      std::copy(out.begin(), out.end(), flows + (size_t)f * 3 * n);
+    }
+  }
+  if (L.do_warp) {
+    for (int f = 0; f < L.nstates; ++f) {
+      for (int ik = ws[f]; ik <= we[f]; ++ik) {
+        float td = ((float)ik - ftime[f]) / L.timeunit;
+        td = std::min(std::max(-1.0f, td), 1.0f);
+        warp_and_project<R>(L, views[ik], sino + (size_t)ik * nd, f, td, vols, flows, mask);
+      }
+    }
+  }
+}
+
+}  // namespace
+
+// =================================================================================
+// C ABI
+// =================================================================================
+extern "C" {
+
+int orc_version(void) { return 1; }
+
+// PSART::RotateVolume (Project/WaP_Tomo.cpp:1764-1797), Scene::_calculateCamera /
+// _calculateMatrix (Project/Scene.cpp:65-127, 163-189), TypicalRayProducer::
+// _calculateMatrix (Project/TypicalRayProducer.cpp:157-232), EyePos
+// (Project/TypicalRayProducer.core.h:255-258); PSART ctor frustum half sizes
+// (Project/WaP_Tomo.cpp:255-256). The viewport is the detector size.
+void orc_view_geometry(const WapScanGeometry* g, double angle_deg, WapView* out) {
+  const double angle = angle_deg * 3.1415926 / 180;
+  float vy = 1.0f;
+  if (!g->counterclockwise) vy *= -1.0f;
+  const double length = (double)sqrtf(0.0f * 0.0f + vy * vy + 0.0f * 0.0f);
+  const float s = (float)(1.0f / length);
+  const M4 R = quat_matrix(0.0f * s, vy * s, 0.0f * s, (float)(length * angle));
+  const float zbase = -1 * g->sid;
+  const M4 View = m4_mul(m4_mul(m4_translate(g->translate[0], g->translate[1],
+                                             zbase + g->translate[2]),
+                                m4_scale(1.0f, 1.0f, 1.0f)),
+                         R);
+  const double hw = (g->det_u - 1) * g->det_pitch * 0.5f;
+  const double hh = (g->det_v - 1) * g->det_pitch * 0.5f;
+  const float Nr = g->sdd, Fr = 10000.0f;
+  M4 P;
+  for (int i = 0; i < 16; ++i) P.e[i] = 0.0f;
+  P.e[0] = (float)(Nr / hw);
+  P.e[5] = (float)(Nr / hh);
+  P.e[10] = -(Fr + Nr) / (Fr - Nr);
+  P.e[11] = -1.0f;
+  P.e[14] = -2.0f * Fr * Nr / (Fr - Nr);
+  const float sp = g->vol_spacing;
+  const float vs[3] = {(g->vol_dims[0] - 1) * sp, (g->vol_dims[1] - 1) * sp,
+                       (g->vol_dims[2] - 1) * sp};
+  const M4 MV = m4_mul(View, m4_translate(-vs[0] * 0.5f, -vs[1] * 0.5f, -vs[2] * 0.5f));
+  M4 M2V = m4_identity();
+  M2V.e[0] = 1.0f / sp;
+  M2V.e[5] = 1.0f / sp;
+  M2V.e[10] = 1.0f / sp;
+  const M4 V2M = m4_inverse(M2V);
+  M4 VtoP = m4_identity();
+  VtoP.e[0] = 0.5f * (g->det_u - 1);
+  VtoP.e[5] = 0.5f * (g->det_v - 1);
+  VtoP.e[10] = 0.5f;
+  VtoP.e[12] = VtoP.e[0];
+  VtoP.e[13] = VtoP.e[5];
+  VtoP.e[14] = VtoP.e[10];
+  VtoP.e[15] = 1.0f;
+  std::memset(out, 0, sizeof(*out));
+  out->half_w = hw;
+  out->half_h = hh;
+  out->ds = g->det_pitch;
+  out->sdd = g->sdd;
+  out->parallel = g->parallel;
+  std::memcpy(out->m2v, M2V.e, sizeof(M2V.e));
+  const M4 MVV2M = m4_mul(MV, V2M);
+  std::memcpy(out->mvv2m, MVV2M.e, sizeof(MVV2M.e));
+  if (!g->parallel) {
+    const M4 M2P = m4_mul(m4_mul(VtoP, P), MV);
+    const M4 P2M = m4_inverse(M2P);
+    std::memcpy(out->p2m, P2M.e, sizeof(P2M.e));
+    out->eye[0] = P2M.e[8] / P2M.e[11];
+    out->eye[1] = P2M.e[9] / P2M.e[11];
+    out->eye[2] = P2M.e[10] / P2M.e[11];
+    out->eye[3] = 1.0f;
+  } else {
+    // pixel (i, j) -> view point (i*ds - hw, j*ds - hh, 0) -> model via MV^-1
+    M4 Pix2View = m4_identity();
+    Pix2View.e[0] = (float)g->det_pitch;
+    Pix2View.e[5] = (float)g->det_pitch;
+    Pix2View.e[12] = (float)-hw;
+    Pix2View.e[13] = (float)-hh;
+    const M4 iMV = m4_inverse(MV);
+    const M4 P2M = m4_mul(iMV, Pix2View);
+    std::memcpy(out->p2m, P2M.e, sizeof(P2M.e));
+    float d[4];
+    const float dv[4] = {0.0f, 0.0f, -1.0f, 0.0f};
+    for (int i = 0; i < 4; ++i)
+      d[i] = iMV.e[i] * dv[0] + iMV.e[4 + i] * dv[1] + iMV.e[8 + i] * dv[2] +
+             iMV.e[12 + i] * dv[3];
+    const float li = 1.0f / sqrtf(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+    out->dir[0] = d[0] * li;
+    out->dir[1] = d[1] * li;
+    out->dir[2] = d[2] * li;
+    out->dir[3] = 0.0f;
+  }
+}
+
+// Interleaved view order b, e, b+1, e-1, ... (Project/WaP_Tomo.cpp:1457-1484)
+void orc_view_order(int first, int last, int* order) {
+  const int num = last - first + 1;
+  for (int i = 0; i < num; ++i) {
+    if (2 * i < num) order[2 * i] = first + i;
+    if (2 * i + 1 < num) order[2 * i + 1] = last - i;
+  }
+}
+
+// Frame windows, Project/VolumeData.cpp:156-204 (level 0; base = 3)
+void orc_frame_windows(int nframes, int total_views, float base, float* ftime, int* start,
+                       int* end) {
+  const float timeunit = (float)((float)total_views / (float)nframes);
+  for (int i = 0; i < nframes; ++i) {
+    ftime[i] = timeunit * i;
+    start[i] = std::max(0, (int)(ftime[i] - base));
+    end[i] = std::min((int)(ftime[i] + base), total_views - 1);
+  }
+}
+
+// imROI, Project/WaP_Tomo.cpp:391-417, from explicit m_roi[0..7]
+void orc_roi_mask(const int* dims, const int* r, float* mask) {
+  const int W = dims[0], H = dims[1], D = dims[2];
+  for (int k = 0; k < D; ++k)
+    for (int j = 0; j < H; ++j)
+      for (int i = 0; i < W; ++i) {
+        float m = 1.0f;
+        if ((i < r[6] && k < r[7]) || (i < r[6] && k > D - r[7])) m = 0.0f;
+        if ((i > W - r[6] && k < r[7]) || (i > W - r[6] && k > D - r[7])) m = 0.0f;
+        if ((k < r[6] && i < r[7]) || (k < r[6] && i > W - r[7])) m = 0.0f;
+        if ((k > D - r[6] && i < r[7]) || (k > D - r[6] && i > W - r[7])) m = 0.0f;
+        if (i < r[0] || i > W - r[1] || j < r[2] || j > H - r[3] || k < r[4] || k > D - r[5])
+          m = 0.0f;
+        mask[(size_t)i + (size_t)W * ((size_t)j + (size_t)H * k)] = m;
+      }
+}
+
+void orc_forward_project_f32(const float* vol, const int* dims, float sd, const WapView* v,
+                             int nu, int nv, float* proj, float* raylen, float* weight) {
+  forward_project<float>(vol, dims, sd, *v, nu, nv, proj, raylen, weight);
+}
+
+void orc_forward_project_f64(const double* vol, const int* dims, double sd, const WapView* v,
+                             int nu, int nv, double* proj, double* raylen, double* weight) {
+  forward_project<double>(vol, dims, sd, *v, nu, nv, proj, raylen, weight);
+}
+
+void orc_produce_rays(const int* dims, const WapView* v, int nu, int nv, float* zero4,
+                      float* unit4, float* tn, float* tf) {
+  for (int j = 0; j < nv; ++j)
+    for (int i = 0; i < nu; ++i) {
+      const int k = i + j * nu;
+      Ray<float> r = produce_ray<float>(*v, dims, i, j);
+      for (int c = 0; c < 4; ++c) {
+        zero4[4 * k + c] = r.zero[c];
+        unit4[4 * k + c] = r.unit[c];
+      }
+      tn[k] = r.tnear;
+      tf[k] = r.tfar;
+    }
+}
+
+void orc_backproject_ray_f32(const float* corr, const int* dims, float sd, const WapView* v,
+                             int nu, int nv, float scale, float* out, float* wsum) {
+  backproject_ray<float>(corr, dims, sd, *v, nu, nv, scale, out, wsum);
+}
+
+void orc_backproject_ray_f64(const double* corr, const int* dims, double sd, const WapView* v,
+                             int nu, int nv, double scale, double* out, double* wsum) {
+  backproject_ray<double>(corr, dims, sd, *v, nu, nv, scale, out, wsum);
+}
+
+void orc_sart_correct_f32(const float* b, const float* ax, const float* raylen, float* py,
+                          double alpha, float* corr, int n) {
+  sart_correct<float>(b, ax, raylen, py, alpha, corr, (size_t)n);
+}
+
+void orc_backproject_voxel_f32(float* vol, const int* dims, const float* corr, const WapView* v,
+                               int nu, int nv, double alpha, float gmax, const int* lo,
+                               const int* hi, int mode) {
+  backproject_voxel<float>(vol, dims, corr, *v, nu, nv, alpha, gmax, lo, hi, mode);
+}
+
+void orc_backproject_voxel_f64(double* vol, const int* dims, const double* corr,
+                               const WapView* v, int nu, int nv, double alpha, double gmax,
+                               const int* lo, const int* hi, int mode) {
+  backproject_voxel<double>(vol, dims, corr, *v, nu, nv, alpha, gmax, lo, hi, mode);
+}
+
+void orc_advect_f32(const int* dims, const float* aabb, const float* vel, const float* src,
+                    int nc, float dt, float* out) {
+  advect<float>(dims, aabb, vel, src, nc, dt, out);
+}
+
+void orc_advect_f64(const int* dims, const double* aabb, const double* vel, const double* src,
+                    int nc, double dt, double* out) {
+  advect<double>(dims, aabb, vel, src, nc, dt, out);
+}
+
+void orc_gradient_f32(const float* src, const int* dims, float* out3) {
+  gradient<float>(src, dims, out3);
+}
+
+void orc_blur_f32(const float* src, const int* dims, int nc, float sigma, float* out) {
+  blur<float>(src, dims, nc, sigma, out);
+}
+
+void orc_blur_f64(const double* src, const int* dims, int nc, double sigma, double* out) {
+  blur<double>(src, dims, nc, sigma, out);
+}
+
+void orc_resize_f32(const float* src, const int* dims, int nc, const int* ndims, float* out) {
+  resize<float>(src, dims, nc, ndims, out);
+}
+
+void orc_resize_f64(const double* src, const int* dims, int nc, const int* ndims, double* out) {
+  resize<double>(src, dims, nc, ndims, out);
+}
+
+void orc_flow_single_f32(const int* dims, const float* I1, const float* I2, float* V, float* Y1,
+                         float* Y2, float* Y3, float* Y4, float tol, float lambda, int max_iter,
+                         int nwarps, const float* aabb, const int* roi, int* iters_run) {
+  flow_single<float>(dims, I1, I2, V, Y1, Y2, Y3, Y4, tol, lambda, max_iter, nwarps, aabb, roi,
+                     iters_run);
+}
+
+void orc_flow_multiscale_f32(const int* dims, const float* I1, const float* I2,
+                             const float* V_init, float tol, float lambda, int max_iter,
+                             int nwarps, float eta, float sigma, int scales, const int* cut,
+                             float* V_out) {
+  flow_multiscale<float>(dims, I1, I2, V_init, tol, lambda, max_iter, nwarps, eta, sigma,
+                         scales, cut, V_out);
+}
+
+void orc_flow_multiscale_f64(const int* dims, const double* I1, const double* I2,
+                             const double* V_init, double tol, double lambda, int max_iter,
+                             int nwarps, double eta, double sigma, int scales, const int* cut,
+                             double* V_out) {
+  flow_multiscale<double>(dims, I1, I2, V_init, tol, lambda, max_iter, nwarps, eta, sigma,
+                          scales, cut, V_out);
+}
+
+// TV / temporal-prior step of one frame, Project/WaP_Tomo.cpp:586-660 (Y_bc and the
+// warp terms are empty images, whose arithmetic is a no-op in CImg)
+void orc_tvtp_step_f32(int frame, const int* dims, const int* roi6, float* vol, float* xprev,
+                       float* xbar_f, const float* xbar_next, float* ytv_f, float* ytp_f,
+                       const float* ytp_prev, float cp_sigma, float cp_tau, float tv_w,
+                       float tp_w, float gmax) {
+  const Grid g{dims[0], dims[1], dims[2]};
+  const size_t n = (size_t)g.W * g.H * g.D;
+  (void)xbar_f;
+  for (size_t t = 0; t < n; ++t) ytp_f[t] += (float)(cp_sigma * (xbar_next[t] - xbar_f[t]));
+  const int i0 = roi6[0], i1 = g.W - roi6[1], j0 = roi6[2], j1 = g.H - roi6[3], k0 = roi6[4],
+            k1 = g.D - roi6[5];
+  for (int k = k0; k < k1; ++k)
+    for (int j = j0; j < j1; ++j)
+      for (int i = i0; i < i1; ++i) {
+        const size_t t = g.idx(i, j, k);
+        const float xb = xbar_f[t];
+        ytv_f[t] += cp_sigma * (xbar_f[g.idx(i + 1, j, k)] - xb) * 1.0f;
+        ytv_f[n + t] += cp_sigma * (xbar_f[g.idx(i, j + 1, k)] - xb) * 1.0f;
+        ytv_f[2 * n + t] += cp_sigma * (xbar_f[g.idx(i, j, k + 1)] - xb) * 1.0f;
+        for (int c = 0; c < 3; ++c) ytv_f[c * n + t] = mmax(-tv_w, mmin(tv_w, ytv_f[c * n + t]));
+        ytp_f[t] = ytp_f[t] * tp_w / (tp_w + 2 * cp_sigma);
+      }
+  for (size_t t = 0; t < n; ++t) xprev[t] = vol[t];
+  if (frame == 0) {
+    for (size_t t = 0; t < n; ++t) vol[t] -= cp_tau * (-ytp_f[t]);
+  } else {
+    for (size_t t = 0; t < n; ++t) vol[t] -= cp_tau * (ytp_prev[t] - ytp_f[t]);
+  }
+  for (int k = k0; k < k1; ++k)
+    for (int j = j0; j < j1; ++j)
+      for (int i = i0; i < i1; ++i) {
+        const size_t t = g.idx(i, j, k);
+        const float y1 = -(ytv_f[t] - ytv_f[g.idx(i - 1, j, k)] + ytv_f[n + t] -
+                           ytv_f[n + g.idx(i, j - 1, k)] + ytv_f[2 * n + t] -
+                           ytv_f[2 * n + g.idx(i, j, k - 1)]);
+        vol[t] -= (cp_tau * y1);
+        vol[t] = mmax(mmin(gmax, vol[t]), 0.0f);
+      }
+}
+
+// X_bar = 2x - x_prev over the ROI, Project/WaP_Tomo.cpp:675-689
+void orc_xbar_update_f32(const int* dims, const int* roi6, const float* vol, const float* xprev,
+                         float* xbar) {
+  const Grid g{dims[0], dims[1], dims[2]};
+  for (int k = roi6[4]; k < g.D - roi6[5]; ++k)
+    for (int j = roi6[2]; j < g.H - roi6[3]; ++j)
+      for (int i = roi6[0]; i < g.W - roi6[1]; ++i) {
+        const size_t t = g.idx(i, j, k);
+        xbar[t] = 2 * vol[t] - xprev[t];
+      }
+}
+
+void orc_wap_iteration_f32(const OrcLoop* L, const WapView* views, const float* sino,
+                           const int* win_start, const int* win_end, const float* ftime,
+                           float* vols, float* flows, const float* roimask) {
+  wap_iteration<float>(*L, views, sino, win_start, win_end, ftime, vols, flows, roimask);
+}
+
+void orc_wap_iteration_f64(const OrcLoop* L, const WapView* views, const double* sino,
+                           const int* win_start, const int* win_end, const float* ftime,
+                           double* vols, double* flows, const double* roimask) {
+  wap_iteration<double>(*L, views, sino, win_start, win_end, ftime, vols, flows, roimask);
+}
+
+}  // extern "C"

@@ -1,0 +1,124 @@
+/* CPU oracle for the warp-and-project loop -- TEST INFRASTRUCTURE ONLY.
+ *
+ * A from-scratch restatement of the reference algorithm used as the checker for the
+ * HIP product path. Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
+ * leg load it. Every function exists in two numeric forms:
+ *   - "f32": faithful to the reference's float/double mix, operation by operation
+ *     (compiled with -ffp-contract=off); this is the parity form.
+ *   - "f64": every value in double; the accuracy yardstick ("FP64 oracle").
+ * Each function cites the reference lines it restates in wap_oracle.cpp.
+ */
+#ifndef WAP_ORACLE_H
+#define WAP_ORACLE_H
+
+#include "../include/wap_types.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+int orc_version(void);
+
+/* geometry */
+void orc_view_geometry(const WapScanGeometry* g, double angle_deg, WapView* out);
+void orc_view_order(int first, int last, int* order);
+void orc_frame_windows(int nframes, int total_views, float base, float* ftime, int* start,
+                       int* end);
+void orc_roi_mask(const int* dims, const int* roi8, float* mask);
+
+/* projector: proj/raylen/weight are Nu*Nv per view */
+void orc_forward_project_f32(const float* vol, const int* dims, float sd, const WapView* v,
+                             int nu, int nv, float* proj, float* raylen, float* weight);
+void orc_forward_project_f64(const double* vol, const int* dims, double sd, const WapView* v,
+                             int nu, int nv, double* proj, double* raylen, double* weight);
+/* rays of one view in voxel space (for tests of the setup) */
+void orc_produce_rays(const int* dims, const WapView* v, int nu, int nv, float* zero4,
+                      float* unit4, float* tn, float* tf);
+
+/* matched ray-driven scatter: out[v] += scale * corr[ray] * w, wsum[v] += w */
+void orc_backproject_ray_f32(const float* corr, const int* dims, float sd, const WapView* v,
+                             int nu, int nv, float scale, float* out, float* wsum);
+void orc_backproject_ray_f64(const double* corr, const int* dims, double sd,
+                             const WapView* v, int nu, int nv, double scale, double* out,
+                             double* wsum);
+
+/* SART correction + dual update; b, ax, raylen, pY, corr are Nu*Nv */
+void orc_sart_correct_f32(const float* b, const float* ax, const float* raylen, float* py,
+                          double alpha, float* corr, int n);
+
+/* voxel-driven bicubic backprojection.
+ * mode 0 (prox): vol += alpha*g, clamp [0, gmax], over box [lo, hi)
+ * mode 1 (delta): delta += g over box [lo, hi) */
+void orc_backproject_voxel_f32(float* vol, const int* dims, const float* corr, const WapView* v,
+                               int nu, int nv, double alpha, float gmax, const int* lo,
+                               const int* hi, int mode);
+void orc_backproject_voxel_f64(double* vol, const int* dims, const double* corr,
+                               const WapView* v, int nu, int nv, double alpha, double gmax,
+                               const int* lo, const int* hi, int mode);
+
+/* advection: vel has 3 planes, src/out nc planes */
+void orc_advect_f32(const int* dims, const float* aabb, const float* vel, const float* src,
+                    int nc, float dt, float* out);
+void orc_advect_f64(const int* dims, const double* aabb, const double* vel, const double* src,
+                    int nc, double dt, double* out);
+
+/* image ops */
+void orc_gradient_f32(const float* src, const int* dims, float* out3);
+void orc_blur_f32(const float* src, const int* dims, int nc, float sigma, float* out);
+void orc_blur_f64(const double* src, const int* dims, int nc, double sigma, double* out);
+void orc_resize_f32(const float* src, const int* dims, int nc, const int* ndims, float* out);
+void orc_resize_f64(const double* src, const int* dims, int nc, const int* ndims, double* out);
+
+/* optical flow */
+void orc_flow_single_f32(const int* dims, const float* I1, const float* I2, float* V, float* Y1,
+                         float* Y2, float* Y3, float* Y4, float tol, float lambda, int max_iter,
+                         int nwarps, const float* aabb, const int* roi, int* iters_run);
+void orc_flow_multiscale_f32(const int* dims, const float* I1, const float* I2,
+                             const float* V_init, float tol, float lambda, int max_iter,
+                             int nwarps, float eta, float sigma, int scales, const int* cut,
+                             float* V_out);
+void orc_flow_multiscale_f64(const int* dims, const double* I1, const double* I2,
+                             const double* V_init, double tol, double lambda, int max_iter,
+                             int nwarps, double eta, double sigma, int scales, const int* cut,
+                             double* V_out);
+
+/* TV / temporal-prior dual + primal step of one frame (volumes are 1 plane; ytv 3
+ * planes; ytp, xbar 1 plane (channel 0 of the reference's 3-plane images)). */
+void orc_tvtp_step_f32(int frame, const int* dims, const int* roi6, float* vol,
+                       float* xprev, float* xbar_f, const float* xbar_next, float* ytv_f,
+                       float* ytp_f, const float* ytp_prev, float cp_sigma, float cp_tau,
+                       float tv_w, float tp_w, float gmax);
+void orc_xbar_update_f32(const int* dims, const int* roi6, const float* vol, const float* xprev,
+                         float* xbar);
+
+/* One whole WaP outer iteration (SART-prox of every state, flow of every adjacent pair,
+ * warp-and-project over every state's window), as driven by PSART::Render. */
+typedef struct OrcLoop {
+  int dims[3];
+  int nstates;
+  int nviews;
+  int nu, nv;
+  float sd;          /* ray step */
+  double alpha;      /* SART relaxation */
+  float gmax;        /* volume clamp */
+  int nb_sart;       /* SART sweeps per state */
+  int roi[8];        /* m_roi: box margins (6) + corner-cut sizes (2) */
+  int cut[8];        /* flow ROI margins at full resolution */
+  float flow_tol, flow_lambda, flow_eta, flow_sigma;
+  int flow_iters, flow_warps, flow_scales;
+  int do_flow, do_warp;
+  float timeunit;
+} OrcLoop;
+
+void orc_wap_iteration_f32(const OrcLoop* L, const WapView* views, const float* sino,
+                           const int* win_start, const int* win_end, const float* ftime,
+                           float* vols, float* flows, const float* roimask);
+void orc_wap_iteration_f64(const OrcLoop* L, const WapView* views, const double* sino,
+                           const int* win_start, const int* win_end, const float* ftime,
+                           double* vols, double* flows, const double* roimask);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif
