@@ -1,0 +1,157 @@
+/* wap: the MI355X (gfx950) warp-and-project reconstruction path, C ABI.
+ *
+ * Drop-in boundary for the reference's hot loop. Every entry point:
+ *   - takes device pointers (planar x-fastest volumes, u-fastest detector images),
+ *     explicit sizes and a hipStream_t passed as void*;
+ *   - is stream-ordered and asynchronous unless stated;
+ *   - returns 0 on success or a negative code (WAP_E_*), with a message from
+ *     wap_last_error(ctx); nothing throws across the ABI.
+ * A context owns per-device scratch (view tables, flow-solver state, pyramid levels);
+ * one context per stream, contexts are not thread-safe (as the reference classes).
+ * The reference interface each entry point replaces is cited beside it (paths relative
+ * to the reference tree).
+ */
+#ifndef WAP_H
+#define WAP_H
+
+#include <stddef.h>
+
+#include "wap_types.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define WAP_OK 0
+#define WAP_E_ARG (-1)     /* invalid argument (sizes, null pointers, ranges) */
+#define WAP_E_HIP (-2)     /* a HIP runtime call failed */
+#define WAP_E_NOMEM (-3)   /* device allocation failed */
+#define WAP_E_STATE (-4)   /* missing prerequisite (e.g. views not uploaded) */
+
+typedef struct WapContext WapContext;
+
+/* Context lifetime. Replaces the file-static projector globals of
+ * Project/TypicalRayProducer.core.h:16-21, Project/RayCaster.core.h:11-20 and
+ * Project/SingleScalarVolume.core.h:9-16,331 with one handle per stream. */
+int wap_ctx_create(WapContext** out, int device);
+int wap_ctx_destroy(WapContext* ctx);
+const char* wap_last_error(const WapContext* ctx);
+int wap_version(void);
+
+/* Host: per-view matrices in the reference's float arithmetic.
+ * Replaces PSART::RotateVolume (Project/WaP_Tomo.cpp:1764-1797) + Scene::_calculateCamera/
+ * _calculateMatrix (Project/Scene.cpp:65-189) + TypicalRayProducer::_calculateMatrix
+ * (Project/TypicalRayProducer.cpp:157-232) + EyePos (Project/TypicalRayProducer.core.h:255-258). */
+int wap_view_geometry(const WapScanGeometry* g, double angle_deg, WapView* out);
+
+/* Host: interleaved SART view order first, last, first+1, last-1, ...
+ * Replaces Project/WaP_Tomo.cpp:1457-1484. */
+int wap_view_order(int first, int last, int* order);
+
+/* Host: reference frame windows (Project/VolumeData.cpp:156-204). */
+int wap_frame_windows(int nframes, int total_views, float base, float* ftime, int* start,
+                      int* end);
+
+/* Host: ROI mask imROI from m_roi[0..7] (Project/WaP_Tomo.cpp:391-417). */
+int wap_roi_mask(const int* dims, const int* roi8, float* host_mask);
+
+/* Upload the view table used by the projector entry points (copied; synchronous). */
+int wap_upload_views(WapContext* ctx, const WapView* host_views, int nviews);
+
+/* Forward projection of views [view0, view0+nviews) of the uploaded table.
+ * vol_stride = 0 projects one volume; otherwise view k reads vol + k*vol_stride.
+ * proj/raylen/weight are nviews*nu*nv (raylen/weight may be NULL).
+ * Replaces GenerateImage_GetCorrections / GenerateImage_MT / ProduceRay / CastingRay
+ * (Project/TypicalRayProducer.core.h:32-268, Project/RayCaster.core.h:42-207). */
+int wap_forward_project(WapContext* ctx, const float* vol, size_t vol_stride, const int* dims,
+                        float sd, int view0, int nviews, int nu, int nv, float* proj,
+                        float* raylen, float* weight, void* stream);
+
+/* Forward projection of one view fused with the SART correction:
+ *   corr = (b - Ax - pY)/(raylen + 1);  pY += alpha*corr   (py may be NULL = zero dual)
+ * Replaces RenderContent + the correction loops of Project/WaP_Tomo.cpp:1521-1555 (prox)
+ * and :1252-1284 (warp). proj/raylen may be NULL. */
+int wap_project_correct(WapContext* ctx, const float* vol, const int* dims, float sd, int view,
+                        int nu, int nv, const float* b, float* py, double alpha, float* corr,
+                        float* proj, float* raylen, void* stream);
+
+/* Voxel-driven bicubic backprojection of corr over the voxel box [lo, hi).
+ * mode 0: vol += alpha*g, clamp to [0, gmax] (Project/WaP_Tomo.cpp:1557-1628)
+ * mode 1: vol += g                          (Project/WaP_Tomo.cpp:1294-1374) */
+int wap_backproject_voxel(WapContext* ctx, float* vol, const int* dims, const float* corr,
+                          int view, int nu, int nv, double alpha, float gmax, const int* lo,
+                          const int* hi, int mode, void* stream);
+
+/* Matched ray-driven backprojection (the transpose of wap_forward_project up to the
+ * factor sd): out = scale * sum_rays corr*w, wsum = sum w, over views [view0, view0+n).
+ * Replaces the backprojection branch of T_VolumeAccess (Project/SingleScalarVolume.core.h:
+ * 85-96,149-199) selected by --bp Raybased (Project/main.cpp:68). wsum may be NULL. */
+int wap_backproject_ray(WapContext* ctx, const float* corr, const int* dims, float sd,
+                        int view0, int nviews, int nu, int nv, double scale, float* out,
+                        float* wsum, void* stream);
+
+/* advection::advect (Project/advection.h:117-146): vel has 3 planes, src/out nc planes. */
+int wap_advect(WapContext* ctx, const int* dims, const float* aabb, const float* vel,
+               const float* src, int nc, float dt, float* out, void* stream);
+
+/* CImg get_blur(sigma) (Deriche, Neumann) of nc planes (Project/CImg.h:33249-33285). */
+int wap_blur(WapContext* ctx, const float* src, const int* dims, int nc, float sigma,
+             float* out, void* stream);
+
+/* CImg get_resize(nw, nh, nd, nc, 5) of nc planes (Project/CImg.h:28434-28610). */
+int wap_resize(WapContext* ctx, const float* src, const int* dims, int nc, const int* ndims,
+               float* out, void* stream);
+
+/* One scale of L1TVOpticalFlowNonlinear3D (Project/optical_flow.h:87-995).
+ * V: 3 planes (in/out), Y: 10 planes y11..y33,y4 (in/out; written back over the ROI),
+ * aabb: world box of the advection, roi: 6 margins. iters_run (host, may be NULL)
+ * receives the iterations executed summed over warps (forces a synchronisation). */
+int wap_flow_solve(WapContext* ctx, const int* dims, const float* I1, const float* I2, float* V,
+                   float* Y, float tol, float lambda, int max_iter, int nwarps,
+                   const float* aabb, const int* roi, int* iters_run, void* stream);
+
+/* L1TVOpticalFlowNonlinear_MultiScale3D (Project/optical_flow.h:1002-1126).
+ * V_init may be NULL (empty flow). V_out: 3 planes at full resolution. */
+int wap_flow_multiscale(WapContext* ctx, const int* dims, const float* I1, const float* I2,
+                        const float* V_init, float tol, float lambda, int max_iter, int nwarps,
+                        float eta, float sigma, int scales, const int* cut, float* V_out,
+                        void* stream);
+
+/* TV / temporal-prior dual + primal step of one frame (Project/WaP_Tomo.cpp:586-660);
+ * ytv 3 planes, ytp/xbar 1 plane. ytp_prev is ignored for frame 0. */
+int wap_tvtp_step(WapContext* ctx, int frame, const int* dims, const int* roi6, float* vol,
+                  float* xprev, const float* xbar_f, const float* xbar_next, float* ytv_f,
+                  float* ytp_f, const float* ytp_prev, float cp_sigma, float cp_tau,
+                  float tv_w, float tp_w, float gmax, void* stream);
+
+/* X_bar = 2x - x_prev over the ROI (Project/WaP_Tomo.cpp:675-689). */
+int wap_xbar_update(WapContext* ctx, const int* dims, const int* roi6, float* xbar,
+                    const float* vol, const float* xprev, void* stream);
+
+/* vol = max(vol*mask, 0) (Project/WaP_Tomo.cpp:1646-1659). */
+int wap_roi_apply(WapContext* ctx, float* vol, const float* mask, size_t n, void* stream);
+
+/* SART_Proximal_Operator of one state over its view window [first, last]
+ * (Project/WaP_Tomo.cpp:1424-1678). sino: nviews*nu*nv (all views); views uploaded. */
+int wap_sart_prox(WapContext* ctx, const WapLoopConfig* cfg, float* vol, const float* sino,
+                  int first, int last, const float* mask, void* stream);
+
+/* WarpAndProjct of one view (Project/WaP_Tomo.cpp:1207-1421): vols nstates*N, flows
+ * nstates*3N (the last state's flow is the zero field). */
+int wap_warp_and_project(WapContext* ctx, const WapLoopConfig* cfg, float* vols,
+                         const float* flows, const float* sino, int frame, int view,
+                         float rtime, const float* mask, void* stream);
+
+/* One whole outer iteration (PSART::Render, Project/WaP_Tomo.cpp:473-877): SART-prox of
+ * every state, multiscale flow of every adjacent pair (flows updated in place, used as
+ * the initial guess), warp-and-project over every state's window. win_start/win_end/
+ * ftime are host arrays of nstates entries. */
+int wap_iteration(WapContext* ctx, const WapLoopConfig* cfg, float* vols, float* flows,
+                  const float* sino, const int* win_start, const int* win_end,
+                  const float* ftime, const float* mask, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* WAP_H */

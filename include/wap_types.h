@@ -48,6 +48,26 @@ typedef struct WapView {
   int parallel;     /* 0 cone, 1 parallel */
 } WapView;
 
+/* One warp-and-project outer iteration (PSART::Render, Project/WaP_Tomo.cpp:473-877):
+ * SART-prox sweeps of every state, the optical flow of every adjacent pair, and a
+ * warp-and-project pass over every state's view window. */
+typedef struct WapLoopConfig {
+  int dims[3];        /* W, H, D */
+  int nstates;        /* deformation states (frames) */
+  int nviews;         /* projections */
+  int nu, nv;         /* detector */
+  float sd;           /* ray step (reference: sqrt(3)/2 * spacing, WaP_Tomo.cpp:380) */
+  double alpha;       /* SART relaxation (reference m_alpha, -a) */
+  float gmax;         /* volume upper clamp (reference G_Max) */
+  int nb_sart;        /* SART sweeps per state (reference nb_proxiSART) */
+  int roi[8];         /* m_roi: box margins x0,x1,y0,y1,z0,z1 + corner-cut sizes */
+  int cut[8];         /* flow ROI margins at full resolution (reference cut[]) */
+  float flow_tol, flow_lambda, flow_eta, flow_sigma;
+  int flow_iters, flow_warps, flow_scales;
+  int do_flow, do_warp;
+  float timeunit;     /* views per state, for the warp time offset */
+} WapLoopConfig;
+
 /* Storage element type of volumes, flows and sinograms. */
 typedef enum WapDType { WAP_F32 = 0, WAP_BF16 = 1, WAP_F16 = 2 } WapDType;
 

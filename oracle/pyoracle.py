@@ -70,6 +70,11 @@ def _c(a, dt):
     return np.ascontiguousarray(a, dtype=dt)
 
 
+def _v(view):
+    """Accept a WapView from any ctypes mirror with the same layout."""
+    return view if isinstance(view, WapView) else WapView.from_buffer_copy(bytes(view))
+
+
 class Oracle:
     def __init__(self, path=ORACLE_SO):
         if not os.path.exists(path):
@@ -155,12 +160,12 @@ class Oracle:
         if f64:
             vol = _c(vol, np.float64)
             out = [np.zeros((nv, nu), np.float64) for _ in range(3)]
-            self.lib.orc_forward_project_f64(vol, _dims(vol.shape), sd, C.byref(view), nu, nv,
+            self.lib.orc_forward_project_f64(vol, _dims(vol.shape), sd, C.byref(_v(view)), nu, nv,
                                              *out)
         else:
             vol = _c(vol, np.float32)
             out = [np.zeros((nv, nu), np.float32) for _ in range(3)]
-            self.lib.orc_forward_project_f32(vol, _dims(vol.shape), sd, C.byref(view), nu, nv,
+            self.lib.orc_forward_project_f32(vol, _dims(vol.shape), sd, C.byref(_v(view)), nu, nv,
                                              *out)
         return tuple(out)
 
@@ -169,7 +174,7 @@ class Oracle:
         u = np.zeros((nv * nu, 4), np.float32)
         tn = np.zeros(nv * nu, np.float32)
         tf = np.zeros(nv * nu, np.float32)
-        self.lib.orc_produce_rays(_dims(shape), C.byref(view), nu, nv, z, u, tn, tf)
+        self.lib.orc_produce_rays(_dims(shape), C.byref(_v(view)), nu, nv, z, u, tn, tf)
         return z, u, tn, tf
 
     def backproject_ray(self, corr, shape, view, sd, scale=1.0, f64=False):
@@ -178,7 +183,7 @@ class Oracle:
         out = np.zeros(shape, dt)
         ws = np.zeros(shape, dt)
         fn = self.lib.orc_backproject_ray_f64 if f64 else self.lib.orc_backproject_ray_f32
-        fn(_c(corr, dt), _dims(shape), sd, C.byref(view), nu, nv, scale, out, ws)
+        fn(_c(corr, dt), _dims(shape), sd, C.byref(_v(view)), nu, nv, scale, out, ws)
         return out, ws
 
     def sart_correct(self, b, ax, raylen, py, alpha):
@@ -193,7 +198,7 @@ class Oracle:
         dt = np.float64 if f64 else np.float32
         vol = _c(vol, dt).copy()
         fn = self.lib.orc_backproject_voxel_f64 if f64 else self.lib.orc_backproject_voxel_f32
-        fn(vol, _dims(vol.shape), _c(corr, dt), C.byref(view), nu, nv, alpha, gmax,
+        fn(vol, _dims(vol.shape), _c(corr, dt), C.byref(_v(view)), nu, nv, alpha, gmax,
            _c(lo, np.int32), _c(hi, np.int32), mode)
         return vol
 

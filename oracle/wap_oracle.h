@@ -93,22 +93,7 @@ void orc_xbar_update_f32(const int* dims, const int* roi6, const float* vol, con
 
 /* One whole WaP outer iteration (SART-prox of every state, flow of every adjacent pair,
  * warp-and-project over every state's window), as driven by PSART::Render. */
-typedef struct OrcLoop {
-  int dims[3];
-  int nstates;
-  int nviews;
-  int nu, nv;
-  float sd;          /* ray step */
-  double alpha;      /* SART relaxation */
-  float gmax;        /* volume clamp */
-  int nb_sart;       /* SART sweeps per state */
-  int roi[8];        /* m_roi: box margins (6) + corner-cut sizes (2) */
-  int cut[8];        /* flow ROI margins at full resolution */
-  float flow_tol, flow_lambda, flow_eta, flow_sigma;
-  int flow_iters, flow_warps, flow_scales;
-  int do_flow, do_warp;
-  float timeunit;
-} OrcLoop;
+typedef WapLoopConfig OrcLoop;
 
 void orc_wap_iteration_f32(const OrcLoop* L, const WapView* views, const float* sino,
                            const int* win_start, const int* win_end, const float* ftime,

@@ -1,0 +1,156 @@
+// Semi-Lagrangian advection of a state by a velocity field, for gfx950.
+//
+// Reference semantics: advection::advect / trace / sample_image / sample_velocity
+// (Project/advection.h:10-146) with CImg's Catmull-Rom _cubic_atXYZ
+// (Project/CImg.h:13429-13503). For the centre p of cell (i,j,k):
+//   p2 = p + dt*v(p);  p0 = p2 - dt*v(p2);  q = p2 + 0.5*(p - p0)
+//   out = clamp(cubic(src, grid(q)), min8, max8), grid clamped to [1, dim-2]
+// where dt is the negated advect time step and min8/max8 span the 8 cells around
+// the sample. The grid<->world maps add and subtract 0.5 in double, as the reference
+// does through its double literals.
+//
+// One thread per output voxel; the three velocity channels share one index/weight
+// computation. The kernel is gather- and ALU-bound (7 clamped tricubics, ~450 taps per
+// voxel), not HBM-bound.
+#include "wap_device.h"
+
+namespace {
+
+struct AdvGrid {
+  int W, H, D;
+  float bb[6];
+};
+
+WAP_DEV float cubic1(float Ip, float Ic, float In, float Ia, float d) {
+  return Ic + 0.5f * (d * (-Ip + In) + d * d * (2.0f * Ip - 5.0f * Ic + 4.0f * In - Ia) +
+                      d * d * d * (-Ip + 3.0f * Ic - 3.0f * In + Ia));
+}
+
+// Tap layout of one clamped Catmull-Rom sample at grid point g (already clamped to
+// [1, dim-2] by the caller, so CImg's cut to [0, dim-1] is a no-op).
+struct Taps {
+  int xs[4];
+  size_t rows[4][4];  // rows[zc][yc] = base offset of the x line
+  float dx, dy, dz;
+  size_t c000;        // offset of (ix, iy, iz) for the 8-cell range
+  int sx, sy, sz;     // +1 strides for the 8-cell range (all in-bounds)
+};
+
+WAP_DEV void clamp_grid(const AdvGrid& G, const float* w, float* g) {
+  const int dim[3] = {G.W, G.H, G.D};
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const float s = (float)dim[a] * (w[a] - G.bb[2 * a]) / (G.bb[2 * a + 1] - G.bb[2 * a]);
+    float v = (float)((double)s - 0.5);
+    const float hi = (float)(dim[a] - 2);
+    const float m = (v < hi) ? v : hi;
+    g[a] = (1.0f < m) ? m : 1.0f;
+  }
+}
+
+WAP_DEV Taps make_taps(const AdvGrid& G, const float* g) {
+  Taps T;
+  const int x = (int)g[0], y = (int)g[1], z = (int)g[2];
+  T.dx = g[0] - (float)x;
+  T.dy = g[1] - (float)y;
+  T.dz = g[2] - (float)z;
+  T.xs[0] = x - 1 < 0 ? 0 : x - 1;
+  T.xs[1] = x;
+  T.xs[2] = T.dx > 0.0f ? x + 1 : x;
+  T.xs[3] = x + 2 >= G.W ? G.W - 1 : x + 2;
+  const int ys[4] = {y - 1 < 0 ? 0 : y - 1, y, T.dy > 0.0f ? y + 1 : y,
+                     y + 2 >= G.H ? G.H - 1 : y + 2};
+  const int zs[4] = {z - 1 < 0 ? 0 : z - 1, z, T.dz > 0.0f ? z + 1 : z,
+                     z + 2 >= G.D ? G.D - 1 : z + 2};
+#pragma unroll
+  for (int c = 0; c < 4; ++c)
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+      T.rows[c][b] = (size_t)G.W * ((size_t)ys[b] + (size_t)G.H * (size_t)zs[c]);
+  T.c000 = vidx(x, y, z, G.W, G.H);
+  T.sx = 1;
+  T.sy = G.W;
+  T.sz = G.W * G.H;
+  return T;
+}
+
+WAP_DEV float sample_taps(const float* __restrict__ img, const Taps& T) {
+  // 8-cell range, in the reference's visiting order
+  const float* c = img + T.c000;
+  float mn = c[0], mx = mn;
+  const int offs[7] = {T.sx, T.sx + T.sy, T.sy, T.sz, T.sx + T.sz, T.sx + T.sy + T.sz,
+                       T.sy + T.sz};
+#pragma unroll
+  for (int q = 0; q < 7; ++q) {
+    const float s = c[offs[q]];
+    mn = (s < mn) ? s : mn;
+    mx = (mx < s) ? s : mx;
+  }
+  float Iz[4];
+#pragma unroll
+  for (int zc = 0; zc < 4; ++zc) {
+    float Iy[4];
+#pragma unroll
+    for (int yc = 0; yc < 4; ++yc) {
+      const float* row = img + T.rows[zc][yc];
+      Iy[yc] = cubic1(row[T.xs[0]], row[T.xs[1]], row[T.xs[2]], row[T.xs[3]], T.dx);
+    }
+    Iz[zc] = cubic1(Iy[0], Iy[1], Iy[2], Iy[3], T.dy);
+  }
+  const float cv = cubic1(Iz[0], Iz[1], Iz[2], Iz[3], T.dz);
+  const float lo = (cv < mx) ? cv : mx;
+  return (mn < lo) ? lo : mn;
+}
+
+__global__ __launch_bounds__(256) void k_advect(AdvGrid G, const float* __restrict__ vel,
+                                                const float* __restrict__ src, int nc, float dt,
+                                                float* __restrict__ out) {
+  const size_t n = (size_t)G.W * G.H * G.D;
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int i = (int)(t % G.W);
+  const int j = (int)((t / G.W) % G.H);
+  const int k = (int)(t / ((size_t)G.W * G.H));
+  const int gi[3] = {i, j, k};
+  const int dim[3] = {G.W, G.H, G.D};
+  float p[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+    p[a] = (float)((double)G.bb[2 * a] + ((double)(float)gi[a] + 0.5) *
+                                             (double)(G.bb[2 * a + 1] - G.bb[2 * a]) /
+                                             (double)(float)dim[a]);
+  const float tdt = -dt;
+  float g[3], s[3], p2[3], p0[3], q[3];
+  clamp_grid(G, p, g);
+  Taps T = make_taps(G, g);
+#pragma unroll
+  for (int c = 0; c < 3; ++c) s[c] = sample_taps(vel + c * n, T);
+#pragma unroll
+  for (int a = 0; a < 3; ++a) p2[a] = p[a] + tdt * s[a];
+  clamp_grid(G, p2, g);
+  T = make_taps(G, g);
+#pragma unroll
+  for (int c = 0; c < 3; ++c) s[c] = sample_taps(vel + c * n, T);
+#pragma unroll
+  for (int a = 0; a < 3; ++a) p0[a] = p2[a] - tdt * s[a];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) q[a] = p2[a] + (float)(0.5 * (double)(p[a] - p0[a]));
+  clamp_grid(G, q, g);
+  T = make_taps(G, g);
+  for (int c = 0; c < nc; ++c) out[t + c * n] = sample_taps(src + c * n, T);
+}
+
+}  // namespace
+
+hipError_t wapk_advect(const int* dims, const float* aabb, const float* vel, const float* src,
+                       int nc, float dt, float* out, hipStream_t s) {
+  AdvGrid G;
+  G.W = dims[0];
+  G.H = dims[1];
+  G.D = dims[2];
+  for (int a = 0; a < 6; ++a) G.bb[a] = aabb[a];
+  const size_t n = (size_t)G.W * G.H * G.D;
+  hipLaunchKernelGGL(k_advect, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, G, vel, src,
+                     nc, dt, out);
+  return hipGetLastError();
+}

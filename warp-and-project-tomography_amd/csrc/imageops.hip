@@ -1,0 +1,348 @@
+// Image operators around the flow solver and the loop driver, for gfx950:
+// Deriche blur, pyramid resize (moving average down, Catmull-Rom up), the ROI/clamp
+// update of a state, and the TV / temporal-prior step.
+//
+// Reference semantics:
+//   blur       CImg::deriche order 0, Neumann starts   Project/CImg.h:32917-33008, 33249-33285
+//   resize     CImg::get_resize(..., 5): case 2 moving average when shrinking
+//              (:28137-28212), case 5 cubic with cumulative offsets when growing (:28434-28520)
+//   update     Project/WaP_Tomo.cpp:1378-1412 (vol += alpha*back; cut(0, G_Max); max(vol*ROI, 0))
+//   TV/TP      Project/WaP_Tomo.cpp:586-689
+#include "wap_device.h"
+
+namespace {
+
+struct Deriche {
+  float a0, a1, a2, a3, b1, b2, coefp, coefn;
+};
+
+// one line per thread: forward recursion into dst, backward recursion adds onto it
+__global__ __launch_bounds__(256) void k_deriche(const float* __restrict__ src,
+                                                 float* __restrict__ dst, int W, int H, int D,
+                                                 int axis, Deriche c) {
+  int N;
+  size_t base, off;
+  const size_t lines = axis == 0 ? (size_t)H * D : (axis == 1 ? (size_t)W * D : (size_t)W * H);
+  const size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (l >= lines) return;
+  if (axis == 0) {
+    N = W;
+    off = 1;
+    base = l * W;
+  } else if (axis == 1) {
+    N = H;
+    off = W;
+    const size_t x = l % W, z = l / W;
+    base = x + (size_t)W * H * z;
+  } else {
+    N = D;
+    off = (size_t)W * H;
+    base = l;
+  }
+  const float* x = src + base;
+  float* y = dst + base;
+  float xp = x[0];
+  float yb = c.coefp * xp, yp = yb;
+  for (int m = 0; m < N; ++m) {
+    const float xc = x[(size_t)m * off];
+    const float yc = c.a0 * xc + c.a1 * xp - c.b1 * yp - c.b2 * yb;
+    y[(size_t)m * off] = yc;
+    xp = xc;
+    yb = yp;
+    yp = yc;
+  }
+  float xn = x[(size_t)(N - 1) * off], xa = xn;
+  float yn = c.coefn * xn, ya = yn;
+  for (int m = N - 1; m >= 0; --m) {
+    const float xc = x[(size_t)m * off];
+    const float yc = c.a2 * xn + c.a3 * xa - c.b1 * yn - c.b2 * ya;
+    xa = xn;
+    xn = xc;
+    ya = yn;
+    yn = yc;
+    y[(size_t)m * off] = y[(size_t)m * off] + yc;
+  }
+}
+
+// Moving-average shrink along one axis. For output t the host table lists the source
+// indices and integer weights in the reference's visiting order: tmp = sum(x*d)/n.
+__global__ __launch_bounds__(256) void k_resize_avg(const float* __restrict__ src,
+                                                    float* __restrict__ dst, int n, int s,
+                                                    size_t inner, size_t outer,
+                                                    const int* __restrict__ tstart,
+                                                    const int* __restrict__ tidx,
+                                                    const float* __restrict__ tw) {
+  // element e = (o, t, i): o over outer lines, t over the new axis, i over inner
+  const size_t total = outer * (size_t)s * inner;
+  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= total) return;
+  const size_t i = e % inner;
+  const int t = (int)((e / inner) % s);
+  const size_t o = e / (inner * s);
+  const float* x = src + o * (size_t)n * inner + i;
+  float acc = 0.0f;
+  for (int q = tstart[t]; q < tstart[t + 1]; ++q) acc += x[(size_t)tidx[q] * inner] * tw[q];
+  dst[e] = acc / (float)n;
+}
+
+// Catmull-Rom growth along one axis with the reference's cumulative offset table
+// (source index p and fraction t per output, accumulated in double on the host).
+__global__ __launch_bounds__(256) void k_resize_cubic(const float* __restrict__ src,
+                                                      float* __restrict__ dst, int n, int s,
+                                                      size_t inner, size_t outer,
+                                                      const int* __restrict__ pidx,
+                                                      const double* __restrict__ frac) {
+  const size_t total = outer * (size_t)s * inner;
+  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= total) return;
+  const size_t i = e % inner;
+  const int xo = (int)((e / inner) % s);
+  const size_t o = e / (inner * s);
+  const float* x = src + o * (size_t)n * inner + i;
+  const int p = pidx[xo];
+  const double t = frac[xo];
+  const double v1 = (double)x[(size_t)p * inner];
+  const double v0 = p > 0 ? (double)x[(size_t)(p - 1) * inner] : v1;
+  const double v2 = p <= n - 2 ? (double)x[(size_t)(p + 1) * inner] : v1;
+  const double v3 = p < n - 2 ? (double)x[(size_t)(p + 2) * inner] : v2;
+  const double val = v1 + 0.5f * (t * (-v0 + v2) + t * t * (2 * v0 - 5 * v1 + 4 * v2 - v3) +
+                                  t * t * t * (-v0 + 3 * v1 - 3 * v2 + v3));
+  const double lo = -3.4028234663852886e38, hi = 3.4028234663852886e38;
+  dst[e] = (float)(val < lo ? lo : (val > hi ? hi : val));
+}
+
+__global__ void k_fill(float* __restrict__ p, size_t n, float v) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) p[t] = v;
+}
+
+__global__ void k_fill_int(int* __restrict__ p, size_t n, int v) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) p[t] = v;
+}
+
+__global__ void k_fill_double(double* __restrict__ p, size_t n) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) p[t] = 0.0;
+}
+
+__global__ void k_copy(float* __restrict__ d, const float* __restrict__ s, size_t n) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) d[t] = s[t];
+}
+
+// copy of the box [lo, hi) of nplanes planes
+__global__ void k_copy_box(float* __restrict__ d, const float* __restrict__ s, int W, int H,
+                           int D, int nplanes, int l0, int l1, int l2, int h0, int h1, int h2) {
+  const int bx = h0 - l0, by = h1 - l1, bz = h2 - l2;
+  const size_t nb = (size_t)bx * by * bz;
+  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nb * nplanes) return;
+  const size_t q = e % nb;
+  const int c = (int)(e / nb);
+  const int i = l0 + (int)(q % bx), j = l1 + (int)((q / bx) % by),
+            k = l2 + (int)(q / ((size_t)bx * by));
+  const size_t t = vidx(i, j, k, W, H) + (size_t)c * W * H * D;
+  d[t] = s[t];
+}
+
+// vol = max(vol * mask, 0)
+__global__ void k_roi_apply(float* __restrict__ vol, const float* __restrict__ mask, size_t n) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const float m = vol[t] * mask[t];
+  vol[t] = mmax(m, 0.0f);
+}
+
+// warp-and-project state update: vol += (float)(alpha*back); cut(0, gmax); max(vol*mask, 0)
+__global__ void k_wap_update(float* __restrict__ vol, const float* __restrict__ back,
+                             double alpha, float gmax, const float* __restrict__ mask, size_t n) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const float s = (float)(alpha * (double)back[t]);
+  float x = vol[t] + s;
+  x = x < 0.0f ? 0.0f : (x > gmax ? gmax : x);
+  const float m = x * mask[t];
+  vol[t] = mmax(m, 0.0f);
+}
+
+// ---- TV / temporal prior (one frame) -------------------------------------------------
+// ytp += sigma*(xbar_next - xbar)  (every voxel)
+__global__ void k_tvtp_ytp(float* __restrict__ ytp, const float* __restrict__ xn,
+                           const float* __restrict__ xb, float sigma, size_t n) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) ytp[t] += sigma * (xn[t] - xb[t]);
+}
+
+// ROI: ytv += sigma*D+ xbar, clip to +-tvw; ytp *= tpw/(tpw + 2 sigma)
+__global__ void k_tvtp_dual(float* __restrict__ ytv, float* __restrict__ ytp,
+                            const float* __restrict__ xb, int W, int H, int D, int l0, int l1,
+                            int l2, int h0, int h1, int h2, float sigma, float tvw, float tpw) {
+  const int bx = h0 - l0, by = h1 - l1, bz = h2 - l2;
+  const size_t nb = (size_t)bx * by * bz;
+  const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nb) return;
+  const int i = l0 + (int)(q % bx), j = l1 + (int)((q / bx) % by),
+            k = l2 + (int)(q / ((size_t)bx * by));
+  const size_t n = (size_t)W * H * D;
+  const size_t t = vidx(i, j, k, W, H);
+  const float x0 = xb[t];
+  float a = ytv[t] + sigma * (xb[vidx(i + 1, j, k, W, H)] - x0) * 1.0f;
+  float b = ytv[n + t] + sigma * (xb[vidx(i, j + 1, k, W, H)] - x0) * 1.0f;
+  float c = ytv[2 * n + t] + sigma * (xb[vidx(i, j, k + 1, W, H)] - x0) * 1.0f;
+  ytv[t] = mmax(-tvw, mmin(tvw, a));
+  ytv[n + t] = mmax(-tvw, mmin(tvw, b));
+  ytv[2 * n + t] = mmax(-tvw, mmin(tvw, c));
+  ytp[t] = ytp[t] * tpw / (tpw + 2.0f * sigma);
+}
+
+// every voxel: xprev = vol; vol -= tau*(ytp_prev - ytp) (frame 0: vol -= tau*(-ytp))
+__global__ void k_tvtp_primal_a(float* __restrict__ vol, float* __restrict__ xprev,
+                                const float* __restrict__ ytp, const float* __restrict__ ytp_prev,
+                                float tau, size_t n) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  xprev[t] = vol[t];
+  if (ytp_prev)
+    vol[t] -= tau * (ytp_prev[t] - ytp[t]);
+  else
+    vol[t] -= tau * (-ytp[t]);
+}
+
+// ROI: vol -= tau*(-div- ytv); clamp to [0, gmax]
+__global__ void k_tvtp_primal_b(float* __restrict__ vol, const float* __restrict__ ytv, int W,
+                                int H, int D, int l0, int l1, int l2, int h0, int h1, int h2,
+                                float tau, float gmax) {
+  const int bx = h0 - l0, by = h1 - l1, bz = h2 - l2;
+  const size_t nb = (size_t)bx * by * bz;
+  const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nb) return;
+  const int i = l0 + (int)(q % bx), j = l1 + (int)((q / bx) % by),
+            k = l2 + (int)(q / ((size_t)bx * by));
+  const size_t n = (size_t)W * H * D;
+  const size_t t = vidx(i, j, k, W, H);
+  const float y1 = -(ytv[t] - ytv[vidx(i - 1, j, k, W, H)] + ytv[n + t] -
+                     ytv[n + vidx(i, j - 1, k, W, H)] + ytv[2 * n + t] -
+                     ytv[2 * n + vidx(i, j, k - 1, W, H)]);
+  float v = vol[t] - tau * y1;
+  vol[t] = mmax(mmin(gmax, v), 0.0f);
+}
+
+// ROI: xbar = 2*vol - xprev
+__global__ void k_xbar(float* __restrict__ xbar, const float* __restrict__ vol,
+                       const float* __restrict__ xprev, int W, int H, int D, int l0, int l1,
+                       int l2, int h0, int h1, int h2) {
+  const int bx = h0 - l0, by = h1 - l1, bz = h2 - l2;
+  const size_t nb = (size_t)bx * by * bz;
+  const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nb) return;
+  const int i = l0 + (int)(q % bx), j = l1 + (int)((q / bx) % by),
+            k = l2 + (int)(q / ((size_t)bx * by));
+  const size_t t = vidx(i, j, k, W, H);
+  xbar[t] = 2.0f * vol[t] - xprev[t];
+}
+
+inline unsigned nblk(size_t n) { return (unsigned)((n + 255) / 256); }
+
+}  // namespace
+
+hipError_t wapk_deriche(const float* src, float* dst, const int* dims, int axis,
+                        const float* coef8, hipStream_t s) {
+  Deriche c{coef8[0], coef8[1], coef8[2], coef8[3], coef8[4], coef8[5], coef8[6], coef8[7]};
+  const size_t lines = axis == 0 ? (size_t)dims[1] * dims[2]
+                                 : (axis == 1 ? (size_t)dims[0] * dims[2] : (size_t)dims[0] * dims[1]);
+  hipLaunchKernelGGL(k_deriche, dim3(nblk(lines)), dim3(256), 0, s, src, dst, dims[0], dims[1],
+                     dims[2], axis, c);
+  return hipGetLastError();
+}
+
+hipError_t wapk_resize_avg(const float* src, float* dst, int n, int sz, size_t inner,
+                           size_t outer, const int* tstart, const int* tidx, const float* tw,
+                           hipStream_t s) {
+  const size_t total = outer * (size_t)sz * inner;
+  hipLaunchKernelGGL(k_resize_avg, dim3(nblk(total)), dim3(256), 0, s, src, dst, n, sz, inner,
+                     outer, tstart, tidx, tw);
+  return hipGetLastError();
+}
+
+hipError_t wapk_resize_cubic(const float* src, float* dst, int n, int sz, size_t inner,
+                             size_t outer, const int* pidx, const double* frac, hipStream_t s) {
+  const size_t total = outer * (size_t)sz * inner;
+  hipLaunchKernelGGL(k_resize_cubic, dim3(nblk(total)), dim3(256), 0, s, src, dst, n, sz, inner,
+                     outer, pidx, frac);
+  return hipGetLastError();
+}
+
+hipError_t wapk_fill(float* p, size_t n, float v, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_fill, dim3(nblk(n)), dim3(256), 0, s, p, n, v);
+  return hipGetLastError();
+}
+
+hipError_t wapk_fill_double(double* p, size_t n, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_fill_double, dim3(nblk(n)), dim3(256), 0, s, p, n);
+  return hipGetLastError();
+}
+
+hipError_t wapk_copy(float* d, const float* src, size_t n, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_copy, dim3(nblk(n)), dim3(256), 0, s, d, src, n);
+  return hipGetLastError();
+}
+
+hipError_t wapk_copy_box(float* d, const float* src, const int* dims, int nplanes, const int* lo,
+                         const int* hi, hipStream_t s) {
+  const size_t nb = (size_t)(hi[0] - lo[0]) * (hi[1] - lo[1]) * (hi[2] - lo[2]) * nplanes;
+  if (nb == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_copy_box, dim3(nblk(nb)), dim3(256), 0, s, d, src, dims[0], dims[1],
+                     dims[2], nplanes, lo[0], lo[1], lo[2], hi[0], hi[1], hi[2]);
+  return hipGetLastError();
+}
+
+hipError_t wapk_roi_apply(float* vol, const float* mask, size_t n, hipStream_t s) {
+  hipLaunchKernelGGL(k_roi_apply, dim3(nblk(n)), dim3(256), 0, s, vol, mask, n);
+  return hipGetLastError();
+}
+
+hipError_t wapk_wap_update(float* vol, const float* back, double alpha, float gmax,
+                           const float* mask, size_t n, hipStream_t s) {
+  hipLaunchKernelGGL(k_wap_update, dim3(nblk(n)), dim3(256), 0, s, vol, back, alpha, gmax, mask,
+                     n);
+  return hipGetLastError();
+}
+
+hipError_t wapk_tvtp(int frame, const int* dims, const int* roi6, float* vol, float* xprev,
+                     const float* xbar_f, const float* xbar_next, float* ytv_f, float* ytp_f,
+                     const float* ytp_prev, float cp_sigma, float cp_tau, float tv_w, float tp_w,
+                     float gmax, hipStream_t s) {
+  const size_t n = (size_t)dims[0] * dims[1] * dims[2];
+  const int lo[3] = {roi6[0], roi6[2], roi6[4]};
+  const int hi[3] = {dims[0] - roi6[1], dims[1] - roi6[3], dims[2] - roi6[5]};
+  const size_t nb = (hi[0] > lo[0] && hi[1] > lo[1] && hi[2] > lo[2])
+                        ? (size_t)(hi[0] - lo[0]) * (hi[1] - lo[1]) * (hi[2] - lo[2])
+                        : 0;
+  hipLaunchKernelGGL(k_tvtp_ytp, dim3(nblk(n)), dim3(256), 0, s, ytp_f, xbar_next, xbar_f,
+                     cp_sigma, n);
+  if (nb)
+    hipLaunchKernelGGL(k_tvtp_dual, dim3(nblk(nb)), dim3(256), 0, s, ytv_f, ytp_f, xbar_f,
+                       dims[0], dims[1], dims[2], lo[0], lo[1], lo[2], hi[0], hi[1], hi[2],
+                       cp_sigma, tv_w, tp_w);
+  hipLaunchKernelGGL(k_tvtp_primal_a, dim3(nblk(n)), dim3(256), 0, s, vol, xprev, ytp_f,
+                     frame == 0 ? nullptr : ytp_prev, cp_tau, n);
+  if (nb)
+    hipLaunchKernelGGL(k_tvtp_primal_b, dim3(nblk(nb)), dim3(256), 0, s, vol, ytv_f, dims[0],
+                       dims[1], dims[2], lo[0], lo[1], lo[2], hi[0], hi[1], hi[2], cp_tau, gmax);
+  return hipGetLastError();
+}
+
+hipError_t wapk_xbar(const int* dims, const int* roi6, float* xbar, const float* vol,
+                     const float* xprev, hipStream_t s) {
+  const int lo[3] = {roi6[0], roi6[2], roi6[4]};
+  const int hi[3] = {dims[0] - roi6[1], dims[1] - roi6[3], dims[2] - roi6[5]};
+  if (!(hi[0] > lo[0] && hi[1] > lo[1] && hi[2] > lo[2])) return hipSuccess;
+  const size_t nb = (size_t)(hi[0] - lo[0]) * (hi[1] - lo[1]) * (hi[2] - lo[2]);
+  hipLaunchKernelGGL(k_xbar, dim3(nblk(nb)), dim3(256), 0, s, xbar, vol, xprev, dims[0], dims[1],
+                     dims[2], lo[0], lo[1], lo[2], hi[0], hi[1], hi[2]);
+  return hipGetLastError();
+}

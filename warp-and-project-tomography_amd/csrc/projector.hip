@@ -1,0 +1,365 @@
+// Ray-driven forward projector, its matched ray-driven scatter (the transpose), the
+// fused SART correction, and the voxel-driven bicubic backprojector, for gfx950.
+//
+// Reference semantics:
+//   ProduceRay            Project/TypicalRayProducer.core.h:32-69 (integer pixel centres :195)
+//   CastingRay            Project/RayCaster.core.h:42-188 (DDA blocks = whole volume)
+//   trilinear / scatter   Project/SingleScalarVolume.core.h:66-202
+//   pixel loop            Project/TypicalRayProducer.core.h:168-224
+//   SART correction       Project/WaP_Tomo.cpp:1530-1555, 1261-1284
+//   voxel BP              Project/WaP_Tomo.cpp:208-222, 1557-1628, 1294-1374
+//
+// Layout: volume planar x-fastest; detector image u-fastest, one Nu*Nv plane per view.
+// One thread traces one detector pixel; a 256-thread workgroup covers a 16x16 pixel
+// tile (each wave an 8x8 patch) so the 64 rays of a wave stay spatially coherent and
+// their trilinear gathers share cache lines.
+#include "wap_device.h"
+
+namespace {
+
+struct RaySeg {
+  float z[3], u[3];
+  float tn, tf;
+};
+
+// Ray entry point, direction and [tnear, tfar] against the box [0, dim-1]^3.
+WAP_DEV RaySeg produce_ray(const WapView& v, int W, int H, int D, int i, int j) {
+  RaySeg r;
+  float mp[4];
+  mat4_apply(v.p2m, (float)i, (float)j, 0.0f, 1.0f, mp);
+  float org[4], dir[4];
+  if (!v.parallel) {
+    const float inv = 1.0f / mp[3];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) mp[k] *= inv;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      org[k] = v.eye[k];
+      dir[k] = mp[k] - org[k];
+    }
+    const float li = 1.0f / sqrtf(dir[0] * dir[0] + dir[1] * dir[1] + dir[2] * dir[2]);
+    dir[0] *= li;
+    dir[1] *= li;
+    dir[2] *= li;
+  } else {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      org[k] = mp[k];
+      dir[k] = v.dir[k];
+    }
+  }
+  float zz[4], uu[4];
+  mat4_apply(v.m2v, org[0], org[1], org[2], org[3], zz);
+  mat4_apply(v.m2v, dir[0], dir[1], dir[2], dir[3], uu);
+  const float hi[3] = {(float)(W - 1), (float)(H - 1), (float)(D - 1)};
+  float tmin[3], tmax[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const float t1 = (0.0f - zz[a]) / uu[a];
+    const float t2 = (hi[a] - zz[a]) / uu[a];
+    tmin[a] = mmin(t1, t2);
+    tmax[a] = mmax(t1, t2);
+    r.z[a] = zz[a];
+    r.u[a] = uu[a];
+  }
+  r.tn = mmax(mmax(mmax(tmin[0], tmin[1]), tmin[2]), 0.0f);
+  r.tf = mmin(mmin(tmax[0], tmax[1]), tmax[2]);
+  return r;
+}
+
+// Fixed-step march in the reference's order: the entry sample is taken twice; the
+// sample positions restart from tnear + ceil((t1-tnear)/sd)*sd at every DDA segment
+// (segments break only at the 0.5 plane of an axis with a negative direction) and
+// accumulate t += sd, pos += step inside a segment.
+template <class F>
+WAP_DEV void march(const RaySeg& r, int W, int H, int D, float sd, F&& visit) {
+  const float tn = r.tn, tf = r.tf;
+  float t1 = tn;
+  float p1[3], step[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    p1[a] = r.z[a] + r.u[a] * t1;
+    step[a] = r.u[a] * sd;
+  }
+  visit(p1[0], p1[1], p1[2]);
+  const int bs[3] = {W, H, D};
+  float tMax[3], tDel[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    tDel[a] = 0.0f;
+    if (r.u[a] == 0.0f) {
+      tMax[a] = tf;
+    } else if (r.u[a] < 0.0f) {
+      const int x = ((int)ceilf(p1[a] - 0.5f) - 1) / bs[a];
+      tMax[a] = ((float)(x * bs[a]) + 0.5f - r.z[a]) / r.u[a];
+      tDel[a] = (float)(-bs[a]) / r.u[a];
+    } else {
+      const int x = (int)floorf(p1[a] - 0.5f) / bs[a];
+      tMax[a] = ((float)((x + 1) * bs[a]) + 0.5f - r.z[a]) / r.u[a];
+      tDel[a] = (float)bs[a] / r.u[a];
+    }
+  }
+  while (t1 < tf) {
+    float tb = mmin(mmin(tMax[0], tMax[1]), tMax[2]);
+    if (tb > tf) tb = tf;
+    float t = ceilf((t1 - tn) / sd) * sd + tn;
+    float px = r.z[0] + r.u[0] * t, py = r.z[1] + r.u[1] * t, pz = r.z[2] + r.u[2] * t;
+    while (t < tb) {
+      visit(px, py, pz);
+      t += sd;
+      px += step[0];
+      py += step[1];
+      pz += step[2];
+    }
+    if (tMax[0] < tMax[1]) {
+      if (tMax[0] < tMax[2])
+        tMax[0] += tDel[0];
+      else
+        tMax[2] += tDel[2];
+    } else {
+      if (tMax[1] < tMax[2])
+        tMax[1] += tDel[1];
+      else
+        tMax[2] += tDel[2];
+    }
+    t1 = tb;
+  }
+}
+
+struct Corner {
+  int i0, i1, i2;
+  float f0, f1, f2;
+};
+
+WAP_DEV Corner clamp_pos(float p0, float p1, float p2, int W, int H, int D) {
+  Corner c;
+  p0 = mmax(mmin(p0, (float)(W - 1)), 0.0f);
+  p1 = mmax(mmin(p1, (float)(H - 1)), 0.0f);
+  p2 = mmax(mmin(p2, (float)(D - 1)), 0.0f);
+  c.i0 = (int)p0;
+  c.i1 = (int)p1;
+  c.i2 = (int)p2;
+  c.f0 = p0 - (float)c.i0;
+  c.f1 = p1 - (float)c.i1;
+  c.f2 = p2 - (float)c.i2;
+  return c;
+}
+
+// Trilinear value, nested as x (pairs) -> y -> z like the reference.
+WAP_DEV float trilinear(const float* __restrict__ vol, float p0, float p1, float p2, int W,
+                        int H, int D) {
+  const Corner c = clamp_pos(p0, p1, p2, W, H, D);
+  const int x0 = c.i0, x1 = min(c.i0 + 1, W - 1);
+  const int y0 = c.i1, y1 = min(c.i1 + 1, H - 1);
+  const int z0 = c.i2, z1 = min(c.i2 + 1, D - 1);
+  const size_t r00 = (size_t)W * ((size_t)y0 + (size_t)H * z0);
+  const size_t r10 = (size_t)W * ((size_t)y1 + (size_t)H * z0);
+  const size_t r01 = (size_t)W * ((size_t)y0 + (size_t)H * z1);
+  const size_t r11 = (size_t)W * ((size_t)y1 + (size_t)H * z1);
+  float g2 = c.f0, g1 = 1.0f - g2;
+  float a = vol[r00 + x0] * g1 + vol[r00 + x1] * g2;
+  float b = vol[r10 + x0] * g1 + vol[r10 + x1] * g2;
+  float cc = vol[r01 + x0] * g1 + vol[r01 + x1] * g2;
+  float d = vol[r11 + x0] * g1 + vol[r11 + x1] * g2;
+  g2 = c.f1;
+  g1 = 1.0f - g2;
+  a = a * g1 + b * g2;
+  cc = cc * g1 + d * g2;
+  g2 = c.f2;
+  g1 = 1.0f - g2;
+  return a * g1 + cc * g2;
+}
+
+WAP_DEV void pixel_of(unsigned bid, unsigned tilesU, int& i, int& j) {
+  const unsigned tu = bid % tilesU, tv = bid / tilesU;
+  const unsigned lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+  i = (int)(tu * 16 + (wave & 1u) * 8 + (lane & 7u));
+  j = (int)(tv * 16 + (wave >> 1) * 8 + (lane >> 3));
+}
+
+// ---------------------------------------------------------------------------------
+// forward projection (+ optional SART correction epilogue)
+// grid: x = 16x16 pixel tiles of one view (XCD remapped), y = view in batch
+// ---------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_project(
+    const float* __restrict__ vol, size_t vol_stride, int W, int H, int D, float sd,
+    const WapView* __restrict__ views, int nu, int nv, float* __restrict__ proj,
+    float* __restrict__ raylen, float* __restrict__ weight, const float* __restrict__ b,
+    float* __restrict__ py, float* __restrict__ corr, double alpha) {
+  const unsigned tilesU = (nu + 15) / 16, tilesV = (nv + 15) / 16;
+  const unsigned bid = xcd_remap(blockIdx.x, tilesU * tilesV);
+  int i, j;
+  pixel_of(bid, tilesU, i, j);
+  if (i >= nu || j >= nv) return;
+  const int view = blockIdx.y;
+  const WapView& v = views[view];
+  const float* V = vol + vol_stride * view;
+  const RaySeg r = produce_ray(v, W, H, D, i, j);
+  float density = 0.0f, sw = 0.0f, len = 0.0f;
+  if (r.tn < r.tf) {
+    len = r.tf - r.tn;
+    march(r, W, H, D, sd, [&](float x, float y, float z) {
+      sw += 1.0f;
+      density += trilinear(V, x, y, z, W, H, D) * sd;
+    });
+  }
+  const size_t k = (size_t)view * nu * nv + (size_t)j * nu + i;
+  if (proj) proj[k] = density;
+  if (raylen) raylen[k] = len;
+  if (weight) weight[k] = sw;
+  if (corr) {
+    // corr = (b - Ax - pY)/(len + 1); pY += alpha*corr (in double, stored float)
+    const float y = py ? py[k] : 0.0f;
+    const float c = (b[k] - density - y) / (len + 1.0f);
+    corr[k] = c;
+    if (py) py[k] = (float)((double)y + alpha * (double)c);
+  }
+}
+
+// ---------------------------------------------------------------------------------
+// matched ray-driven scatter: acc[v] += corr[ray]*w, wacc[v] += w over the exact
+// forward sample set (FP64 atomics, so the sum does not depend on ray order to
+// within double rounding)
+// ---------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_scatter(const float* __restrict__ corr, int W, int H,
+                                                 int D, float sd,
+                                                 const WapView* __restrict__ views, int nu,
+                                                 int nv, double* __restrict__ acc,
+                                                 double* __restrict__ wacc) {
+  const unsigned tilesU = (nu + 15) / 16, tilesV = (nv + 15) / 16;
+  const unsigned bid = xcd_remap(blockIdx.x, tilesU * tilesV);
+  int i, j;
+  pixel_of(bid, tilesU, i, j);
+  if (i >= nu || j >= nv) return;
+  const int view = blockIdx.y;
+  const size_t k = (size_t)view * nu * nv + (size_t)j * nu + i;
+  const RaySeg r = produce_ray(views[view], W, H, D, i, j);
+  if (!(r.tn < r.tf)) return;
+  const float cr = corr[k];
+  march(r, W, H, D, sd, [&](float x, float y, float z) {
+    const Corner c = clamp_pos(x, y, z, W, H, D);
+    const float m0 = 1.0f - c.f0, m1 = 1.0f - c.f1, m2 = 1.0f - c.f2;
+    const float w[8] = {m0 * m1 * m2,     c.f0 * m1 * m2,     m0 * c.f1 * m2,
+                        c.f0 * c.f1 * m2, m0 * m1 * c.f2,     c.f0 * m1 * c.f2,
+                        m0 * c.f1 * c.f2, c.f0 * c.f1 * c.f2};
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int x1 = c.i0 + (q & 1), y1 = c.i1 + ((q >> 1) & 1), z1 = c.i2 + ((q >> 2) & 1);
+      if (x1 >= W || y1 >= H || z1 >= D) continue;  // zero-weight corner past the face
+      const size_t idx = vidx(x1, y1, z1, W, H);
+      atomicAdd(acc + idx, (double)(cr * w[q]));
+      if (wacc) atomicAdd(wacc + idx, (double)w[q]);
+    }
+  });
+}
+
+__global__ void k_scatter_finish(const double* __restrict__ acc, const double* __restrict__ wacc,
+                                 size_t n, double scale, float* __restrict__ out,
+                                 float* __restrict__ wout) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  out[t] = (float)(scale * acc[t]);
+  if (wout) wout[t] = (float)wacc[t];
+}
+
+// ---------------------------------------------------------------------------------
+// voxel-driven bicubic backprojection over the box [lo, hi)
+//   mode 0: vol += alpha*g then clamp to [0, gmax]   (SART proximal step)
+//   mode 1: vol += g                                   (warp-and-project delta)
+// ---------------------------------------------------------------------------------
+WAP_DEV float cubic_bp(float p0, float p1, float p2, float p3, float x) {
+  const float d20 = p2 - p0;
+  const float d12 = p1 - p2;
+  const double xd = (double)x;
+  const double inner =
+      (double)d20 + xd * (2.0 * p0 - 5.0 * p1 + 4.0 * p2 - p3 + xd * (3.0 * (double)d12 + p3 - p0));
+  return (float)(p1 + 0.5 * xd * inner);
+}
+
+__global__ __launch_bounds__(256) void k_bp_voxel(float* __restrict__ vol, int W, int H, int D,
+                                                  const float* __restrict__ corr,
+                                                  const WapView* __restrict__ vp, int nu, int nv,
+                                                  double alpha, float gmax, int lo0, int lo1,
+                                                  int lo2, int hi0, int hi1, int hi2, int mode) {
+  const int bx = hi0 - lo0, by = hi1 - lo1, bz = hi2 - lo2;
+  const size_t nbox = (size_t)bx * by * bz;
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nbox) return;
+  const int x = lo0 + (int)(t % bx);
+  const int y = lo1 + (int)((t / bx) % by);
+  const int z = lo2 + (int)(t / ((size_t)bx * by));
+  const WapView& v = *vp;
+  float det[4];
+  mat4_apply(v.mvv2m, (float)x, (float)y, (float)z, 1.0f, det);
+  float fx, fy;
+  if (!v.parallel) {
+    fx = (float)(v.half_w - (double)((det[0] / det[2]) * v.sdd));
+    fy = (float)(v.half_h - (double)((det[1] / det[2]) * v.sdd));
+  } else {
+    fx = (float)(v.half_w + (double)det[0]);
+    fy = (float)(v.half_h + (double)det[1]);
+  }
+  fx = (float)((double)fx / v.ds);
+  fy = (float)((double)fy / v.ds);
+  const int px = (int)fx, py = (int)fy;
+  const float dx = fx - (float)px, dy = fy - (float)py;
+  if (px < 2 || px >= nu - 2 || py < 2 || py >= nv - 2) return;
+  const float* pc = corr + (size_t)py * nu + px;
+  float arr[4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    const float* col = pc + (a - 1);
+    arr[a] = cubic_bp(col[-nu], col[0], col[nu], col[2 * nu], dy);
+  }
+  const float g = cubic_bp(arr[0], arr[1], arr[2], arr[3], dx);
+  const size_t idx = vidx(x, y, z, W, H);
+  if (mode == 0) {
+    float val = (float)((double)vol[idx] + alpha * (double)g);
+    val = mmin(gmax, val);
+    vol[idx] = mmax(val, 0.0f);
+  } else {
+    vol[idx] += g;
+  }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------------
+// launchers (called from wap_capi.cpp)
+// ---------------------------------------------------------------------------------
+hipError_t wapk_project(const float* vol, size_t vol_stride, const int* dims, float sd,
+                        const WapView* views_dev, int nviews, int nu, int nv, float* proj,
+                        float* raylen, float* weight, const float* b, float* py, float* corr,
+                        double alpha, hipStream_t s) {
+  const unsigned tiles = ((nu + 15) / 16) * ((nv + 15) / 16);
+  hipLaunchKernelGGL(k_project, dim3(tiles, nviews), dim3(256), 0, s, vol, vol_stride, dims[0],
+                     dims[1], dims[2], sd, views_dev, nu, nv, proj, raylen, weight, b, py, corr,
+                     alpha);
+  return hipGetLastError();
+}
+
+hipError_t wapk_scatter(const float* corr, const int* dims, float sd, const WapView* views_dev,
+                        int nviews, int nu, int nv, double* acc, double* wacc, hipStream_t s) {
+  const unsigned tiles = ((nu + 15) / 16) * ((nv + 15) / 16);
+  hipLaunchKernelGGL(k_scatter, dim3(tiles, nviews), dim3(256), 0, s, corr, dims[0], dims[1],
+                     dims[2], sd, views_dev, nu, nv, acc, wacc);
+  return hipGetLastError();
+}
+
+hipError_t wapk_scatter_finish(const double* acc, const double* wacc, size_t n, double scale,
+                               float* out, float* wout, hipStream_t s) {
+  hipLaunchKernelGGL(k_scatter_finish, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, acc,
+                     wacc, n, scale, out, wout);
+  return hipGetLastError();
+}
+
+hipError_t wapk_bp_voxel(float* vol, const int* dims, const float* corr, const WapView* view_dev,
+                         int nu, int nv, double alpha, float gmax, const int* lo, const int* hi,
+                         int mode, hipStream_t s) {
+  const size_t nbox = (size_t)(hi[0] - lo[0]) * (hi[1] - lo[1]) * (hi[2] - lo[2]);
+  if (nbox == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_bp_voxel, dim3((unsigned)((nbox + 255) / 256)), dim3(256), 0, s, vol,
+                     dims[0], dims[1], dims[2], corr, view_dev, nu, nv, alpha, gmax, lo[0], lo[1],
+                     lo[2], hi[0], hi[1], hi[2], mode);
+  return hipGetLastError();
+}

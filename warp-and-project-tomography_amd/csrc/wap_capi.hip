@@ -1,0 +1,753 @@
+// C ABI of the wap library: context, argument validation, and the host-side drivers
+// (SART proximal sweep, optical-flow warps and pyramid, warp-and-project, whole outer
+// iteration) that sequence the gfx950 kernels on one stream.
+//
+// Reference drivers restated here:
+//   SART_Proximal_Operator        Project/WaP_Tomo.cpp:1424-1678
+//   WarpAndProjct                 Project/WaP_Tomo.cpp:1207-1421 (+ SingleScalarVolumePrepare,
+//                                 Project/SingleScalarVolume.core.h:366-442)
+//   L1TVOpticalFlowNonlinear3D    Project/optical_flow.h:87-995 (warp/iteration loop)
+//   ..._MultiScale3D              Project/optical_flow.h:1002-1126
+//   PSART::Render                 Project/WaP_Tomo.cpp:473-877
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/wap.h"
+#include "wap_device.h"
+
+// kernel launchers (projector.hip, advect.hip, flow.hip, imageops.hip)
+hipError_t wapk_project(const float*, size_t, const int*, float, const WapView*, int, int, int,
+                        float*, float*, float*, const float*, float*, float*, double,
+                        hipStream_t);
+hipError_t wapk_scatter(const float*, const int*, float, const WapView*, int, int, int, double*,
+                        double*, hipStream_t);
+hipError_t wapk_scatter_finish(const double*, const double*, size_t, double, float*, float*,
+                               hipStream_t);
+hipError_t wapk_bp_voxel(float*, const int*, const float*, const WapView*, int, int, double,
+                         float, const int*, const int*, int, hipStream_t);
+hipError_t wapk_advect(const int*, const float*, const float*, const float*, int, float, float*,
+                       hipStream_t);
+size_t wapk_flow_primal_blocks(const int*, const int*);
+size_t wapk_flow_dual_blocks(const int*);
+hipError_t wapk_flow_prep(const int*, const int*, const float*, const float*, const float*,
+                          float*, float*, hipStream_t);
+hipError_t wapk_flow_kx4(const int*, const float*, const float*, float*, hipStream_t);
+hipError_t wapk_flow_primal(const int*, const int*, const float*, float*, const float*,
+                            const float*, float*, int, int, double*, const int*, hipStream_t);
+hipError_t wapk_flow_dual(const int*, const int*, const float*, const float*, float*,
+                          const float*, const float*, const float*, int, int, float, float,
+                          double*, int*, int, hipStream_t);
+hipError_t wapk_flow_resid(const double*, int, const double*, int, int, float, int*, float*,
+                           hipStream_t);
+hipError_t wapk_flow_settle(size_t, int, float*, const float*, int*, hipStream_t);
+hipError_t wapk_flow_arm(int*, hipStream_t);
+hipError_t wapk_deriche(const float*, float*, const int*, int, const float*, hipStream_t);
+hipError_t wapk_resize_avg(const float*, float*, int, int, size_t, size_t, const int*,
+                           const int*, const float*, hipStream_t);
+hipError_t wapk_resize_cubic(const float*, float*, int, int, size_t, size_t, const int*,
+                             const double*, hipStream_t);
+hipError_t wapk_fill(float*, size_t, float, hipStream_t);
+hipError_t wapk_fill_double(double*, size_t, hipStream_t);
+hipError_t wapk_copy(float*, const float*, size_t, hipStream_t);
+hipError_t wapk_copy_box(float*, const float*, const int*, int, const int*, const int*,
+                         hipStream_t);
+hipError_t wapk_roi_apply(float*, const float*, size_t, hipStream_t);
+hipError_t wapk_wap_update(float*, const float*, double, float, const float*, size_t,
+                           hipStream_t);
+hipError_t wapk_tvtp(int, const int*, const int*, float*, float*, const float*, const float*,
+                     float*, float*, const float*, float, float, float, float, float,
+                     hipStream_t);
+hipError_t wapk_xbar(const int*, const int*, float*, const float*, const float*, hipStream_t);
+
+namespace {
+
+struct DevBuf {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+};
+
+struct ResizeTable {
+  int kind = 0;  // 1 = moving average, 2 = cubic
+  int* tstart = nullptr;
+  int* tidx = nullptr;
+  float* tw = nullptr;
+  int* pidx = nullptr;
+  double* frac = nullptr;
+};
+
+}  // namespace
+
+struct WapContext {
+  int device = 0;
+  std::string err;
+  WapView* views = nullptr;
+  int nviews = 0;
+  std::map<std::string, DevBuf> bufs;
+  std::map<std::pair<int, int>, ResizeTable> tables;
+};
+
+namespace {
+
+int fail(WapContext* c, int code, const std::string& msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+#define WAP_HIP(ctx, call)                                                             \
+  do {                                                                                 \
+    hipError_t e_ = (call);                                                            \
+    if (e_ != hipSuccess)                                                              \
+      return fail(ctx, WAP_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+#define WAP_TRY(call)      \
+  do {                     \
+    int r_ = (call);       \
+    if (r_ != WAP_OK) return r_; \
+  } while (0)
+
+// Named scratch that grows on demand (reallocation synchronises the device).
+template <class T>
+int scratch(WapContext* c, const std::string& name, size_t count, T** out) {
+  DevBuf& b = c->bufs[name];
+  const size_t need = std::max<size_t>(count * sizeof(T), 256);
+  if (b.bytes < need) {
+    if (b.ptr) {
+      (void)hipDeviceSynchronize();
+      (void)hipFree(b.ptr);
+      b.ptr = nullptr;
+      b.bytes = 0;
+    }
+    if (hipMalloc(&b.ptr, need) != hipSuccess)
+      return fail(c, WAP_E_NOMEM, "device allocation failed for scratch '" + name + "'");
+    b.bytes = need;
+  }
+  *out = (T*)b.ptr;
+  return WAP_OK;
+}
+
+bool dims_ok(const int* d) { return d && d[0] > 0 && d[1] > 0 && d[2] > 0; }
+size_t nvox(const int* d) { return (size_t)d[0] * d[1] * d[2]; }
+hipStream_t S(void* s) { return (hipStream_t)s; }
+
+// --- resize tables (host-built, cached per (n, s)) ------------------------------------
+int get_table(WapContext* c, int n, int s, const ResizeTable** out) {
+  auto key = std::make_pair(n, s);
+  auto it = c->tables.find(key);
+  if (it != c->tables.end()) {
+    *out = &it->second;
+    return WAP_OK;
+  }
+  ResizeTable t;
+  if (n > s) {
+    // moving average: walk the reference's (a, b, c) counters and record, per output t,
+    // the contributing source indices and integer weights in visiting order
+    std::vector<int> start(s + 1, 0), idx;
+    std::vector<float> w;
+    unsigned a = (unsigned)n * s, b = n, cc = s, si = 0, tt = 0;
+    while (a) {
+      const unsigned d = std::min(b, cc);
+      a -= d;
+      b -= d;
+      cc -= d;
+      idx.push_back((int)si);
+      w.push_back((float)d);
+      if (!b) {
+        ++tt;
+        start[tt] = (int)idx.size();
+        b = n;
+      }
+      if (!cc) {
+        ++si;
+        cc = s;
+      }
+    }
+    t.kind = 1;
+    if (hipMalloc(&t.tstart, sizeof(int) * (s + 1)) != hipSuccess ||
+        hipMalloc(&t.tidx, sizeof(int) * idx.size()) != hipSuccess ||
+        hipMalloc(&t.tw, sizeof(float) * w.size()) != hipSuccess)
+      return fail(c, WAP_E_NOMEM, "resize table allocation failed");
+    WAP_HIP(c, hipMemcpy(t.tstart, start.data(), sizeof(int) * (s + 1), hipMemcpyHostToDevice));
+    WAP_HIP(c, hipMemcpy(t.tidx, idx.data(), sizeof(int) * idx.size(), hipMemcpyHostToDevice));
+    WAP_HIP(c, hipMemcpy(t.tw, w.data(), sizeof(float) * w.size(), hipMemcpyHostToDevice));
+  } else {
+    // cubic: cumulative double offsets, fx = (n-1)/(s-1)
+    std::vector<int> p(s);
+    std::vector<double> fr(s);
+    const double f = s > 1 ? (n - 1.0) / (s - 1) : 0;
+    double curr = 0, old = 0;
+    long pos = 0;
+    for (int x = 0; x < s; ++x) {
+      fr[x] = curr - (unsigned)curr;
+      p[x] = (int)pos;
+      old = curr;
+      curr = std::min(n - 1.0, curr + f);
+      pos += (long)((unsigned)curr - (unsigned)old);
+    }
+    t.kind = 2;
+    if (hipMalloc(&t.pidx, sizeof(int) * s) != hipSuccess ||
+        hipMalloc(&t.frac, sizeof(double) * s) != hipSuccess)
+      return fail(c, WAP_E_NOMEM, "resize table allocation failed");
+    WAP_HIP(c, hipMemcpy(t.pidx, p.data(), sizeof(int) * s, hipMemcpyHostToDevice));
+    WAP_HIP(c, hipMemcpy(t.frac, fr.data(), sizeof(double) * s, hipMemcpyHostToDevice));
+  }
+  c->tables[key] = t;
+  *out = &c->tables[key];
+  return WAP_OK;
+}
+
+// one axis: src (dims) -> dst (dims with axis replaced by s)
+int resize_axis(WapContext* c, const float* src, float* dst, const int* dims, int nc, int axis,
+                int s, hipStream_t st) {
+  const int n = dims[axis];
+  size_t inner = 1, outer = (size_t)nc;
+  for (int a = 0; a < axis; ++a) inner *= dims[a];
+  for (int a = axis + 1; a < 3; ++a) outer *= dims[a];
+  if (n == s) {
+    WAP_HIP(c, wapk_copy(dst, src, inner * outer * n, st));
+    return WAP_OK;
+  }
+  if (n == 1) {
+    // nearest from a single sample: replicate
+    for (int k = 0; k < s; ++k)
+      for (size_t o = 0; o < outer; ++o)
+        WAP_HIP(c, hipMemcpyAsync(dst + (o * s + k) * inner, src + o * inner,
+                                  sizeof(float) * inner, hipMemcpyDeviceToDevice, st));
+    return WAP_OK;
+  }
+  const ResizeTable* t = nullptr;
+  WAP_TRY(get_table(c, n, s, &t));
+  if (t->kind == 1)
+    WAP_HIP(c, wapk_resize_avg(src, dst, n, s, inner, outer, t->tstart, t->tidx, t->tw, st));
+  else
+    WAP_HIP(c, wapk_resize_cubic(src, dst, n, s, inner, outer, t->pidx, t->frac, st));
+  return WAP_OK;
+}
+
+int resize_impl(WapContext* c, const float* src, const int* dims, int nc, const int* nd,
+                float* out, hipStream_t st) {
+  float *t1, *t2;
+  const size_t n1 = (size_t)nd[0] * dims[1] * dims[2] * nc;
+  const size_t n2 = (size_t)nd[0] * nd[1] * dims[2] * nc;
+  WAP_TRY(scratch(c, "resize_t1", n1, &t1));
+  WAP_TRY(scratch(c, "resize_t2", n2, &t2));
+  const int d1[3] = {nd[0], dims[1], dims[2]};
+  const int d2[3] = {nd[0], nd[1], dims[2]};
+  WAP_TRY(resize_axis(c, src, t1, dims, nc, 0, nd[0], st));
+  WAP_TRY(resize_axis(c, t1, t2, d1, nc, 1, nd[1], st));
+  WAP_TRY(resize_axis(c, t2, out, d2, nc, 2, nd[2], st));
+  return WAP_OK;
+}
+
+void deriche_coef(float sigma, float* k) {
+  const float ns = sigma < 0.1f ? 0.1f : sigma;
+  const float alpha = 1.695f / ns;
+  const float ema = expf(-alpha), ema2 = expf(-2 * alpha);
+  const float b1 = -2 * ema, b2 = ema2;
+  const float kk = (1 - ema) * (1 - ema) / (1 + 2 * alpha * ema - ema2);
+  const float a0 = kk, a1 = kk * (alpha - 1) * ema, a2 = kk * (alpha + 1) * ema, a3 = -kk * ema2;
+  k[0] = a0;
+  k[1] = a1;
+  k[2] = a2;
+  k[3] = a3;
+  k[4] = b1;
+  k[5] = b2;
+  k[6] = (a0 + a1) / (1 + b1 + b2);
+  k[7] = (a2 + a3) / (1 + b1 + b2);
+}
+
+int blur_impl(WapContext* c, const float* src, const int* dims, int nc, float sigma, float* out,
+              hipStream_t st) {
+  const size_t n = nvox(dims);
+  if (sigma < 0.1f) {
+    WAP_HIP(c, wapk_copy(out, src, n * nc, st));
+    return WAP_OK;
+  }
+  float coef[8];
+  deriche_coef(sigma, coef);
+  float* tmp;
+  WAP_TRY(scratch(c, "blur_tmp", n, &tmp));
+  for (int ch = 0; ch < nc; ++ch) {
+    const float* s = src + ch * n;
+    float* o = out + ch * n;
+    // the axes that run: x, y, z when their extent exceeds 1; ping-pong so the last
+    // pass lands in o
+    int axes[3], na = 0;
+    for (int a = 0; a < 3; ++a)
+      if (dims[a] > 1) axes[na++] = a;
+    if (na == 0) {
+      WAP_HIP(c, wapk_copy(o, s, n, st));
+      continue;
+    }
+    const float* cur = s;
+    for (int q = 0; q < na; ++q) {
+      float* dst = ((na - 1 - q) % 2 == 0) ? o : tmp;
+      WAP_HIP(c, wapk_deriche(cur, dst, dims, axes[q], coef, st));
+      cur = dst;
+    }
+  }
+  return WAP_OK;
+}
+
+// ---------------------------------------------------------------------------------
+// optical flow: one scale (the warp/iteration loop)
+// ---------------------------------------------------------------------------------
+int flow_level(WapContext* c, const int* dims, const float* I1, const float* I2, float* V,
+               float* Y, float tol, float lambda, int max_iter, int nwarps, const float* aabb,
+               const int* roi, int* iters_run, hipStream_t st) {
+  const size_t n = nvox(dims);
+  const int lo[3] = {roi[0], roi[2], roi[4]};
+  const int hi[3] = {dims[0] - roi[1], dims[1] - roi[3], dims[2] - roi[5]};
+  const bool has_roi = hi[0] > lo[0] && hi[1] > lo[1] && hi[2] > lo[2];
+  float *vB, *yW, *u, *ut, *warped, *kty, *kx4;
+  double *pp, *dp;
+  int* ctl;
+  float* err;
+  const size_t npb = std::max<size_t>(wapk_flow_primal_blocks(dims, roi), 1);
+  const size_t ndb = wapk_flow_dual_blocks(dims);
+  WAP_TRY(scratch(c, "flow_vB", 3 * n, &vB));
+  WAP_TRY(scratch(c, "flow_y", 10 * n, &yW));
+  WAP_TRY(scratch(c, "flow_u", 3 * n, &u));
+  WAP_TRY(scratch(c, "flow_ut", n, &ut));
+  WAP_TRY(scratch(c, "flow_warped", n, &warped));
+  WAP_TRY(scratch(c, "flow_kty", 3 * n, &kty));
+  WAP_TRY(scratch(c, "flow_kx4", n, &kx4));
+  WAP_TRY(scratch(c, "flow_pp", npb, &pp));
+  WAP_TRY(scratch(c, "flow_dp", ndb, &dp));
+  WAP_TRY(scratch(c, "flow_ctl", 4, &ctl));
+  WAP_TRY(scratch(c, "flow_err", 4, &err));
+  // state: v in V (A) and vB; y evolves in yW everywhere, written back over the ROI
+  WAP_HIP(c, wapk_copy(vB, V, 3 * n, st));
+  WAP_HIP(c, wapk_copy(yW, Y, 10 * n, st));
+  WAP_HIP(c, wapk_fill(u, 3 * n, 0.0f, st));
+  WAP_HIP(c, wapk_fill(ut, n, 0.0f, st));
+  WAP_HIP(c, wapk_fill(kty, 3 * n, 0.0f, st));
+  const float sigma1 = std::min(1.0f / 3.0f, std::min(1.0f / 3.0f, 1.0f / 3.0f));
+  float* vcur = V;
+  float* vnext = vB;
+  int total = 0;
+  std::vector<int> ctl_host(4);
+  for (int w = 0; w < nwarps; ++w) {
+    WAP_HIP(c, wapk_flow_arm(ctl, st));
+    if (w > 0) WAP_HIP(c, wapk_flow_kx4(dims, u, vcur, kx4, st));
+    WAP_HIP(c, wapk_advect(dims, aabb, vcur, I2, 1, -1.0f, warped, st));
+    if (has_roi) WAP_HIP(c, wapk_flow_prep(dims, roi, warped, I1, vcur, u, ut, st));
+    const int last = max_iter + 1;
+    for (int it = 1; it <= last; ++it) {
+      const int mode = (it == 1) ? (w == 0 ? 0 : 1) : 2;
+      const int resid = (it % 50 == 0);
+      const int store = (it % 50 == 49);
+      if (has_roi)
+        WAP_HIP(c, wapk_flow_primal(dims, roi, vcur, vnext, yW, u, kty, store, resid, pp, ctl,
+                                    st));
+      else
+        WAP_HIP(c, wapk_copy(vnext, vcur, 3 * n, st));
+      WAP_HIP(c, wapk_flow_dual(dims, roi, vcur, vnext, yW, u, ut, kx4, mode, resid, sigma1,
+                                lambda, dp, ctl, it, st));
+      if (resid) {
+        if (!has_roi) WAP_HIP(c, wapk_fill_double(pp, npb, st));
+        WAP_HIP(c, wapk_flow_resid(pp, has_roi ? (int)wapk_flow_primal_blocks(dims, roi) : 1, dp,
+                                   (int)ndb, (int)n, tol, ctl, err, st));
+      }
+      std::swap(vcur, vnext);
+    }
+    WAP_HIP(c, wapk_flow_settle(3 * n, last, vcur, vnext, ctl, st));
+    if (iters_run) {
+      WAP_HIP(c, hipMemcpyAsync(ctl_host.data(), ctl, sizeof(int) * 2, hipMemcpyDeviceToHost, st));
+      WAP_HIP(c, hipStreamSynchronize(st));
+      total += ctl_host[1];
+    }
+  }
+  if (vcur != V) WAP_HIP(c, wapk_copy(V, vcur, 3 * n, st));
+  if (has_roi) WAP_HIP(c, wapk_copy_box(Y, yW, dims, 10, lo, hi, st));
+  if (iters_run) *iters_run = total;
+  return WAP_OK;
+}
+
+int flow_multiscale_impl(WapContext* c, const int* dims, const float* I1, const float* I2,
+                         const float* V_init, float tol, float lambda, int max_iter, int nwarps,
+                         float eta, float sigma, int scales, const int* cut, float* V_out,
+                         hipStream_t st) {
+  const size_t n0 = nvox(dims);
+  struct Lv {
+    int d[3];
+    size_t n, off;
+  };
+  std::vector<Lv> lv(scales);
+  float t_eta = 1.0f;
+  size_t total = 0;
+  for (int s = 0; s < scales; ++s) {
+    Lv& L = lv[scales - 1 - s];
+    for (int a = 0; a < 3; ++a) L.d[a] = (int)((float)dims[a] * t_eta);
+    if (L.d[0] < 4 || L.d[1] < 4 || L.d[2] < 4)
+      return fail(c, WAP_E_ARG, "flow pyramid level smaller than 4 voxels; use fewer scales");
+    L.n = nvox(L.d);
+    t_eta = t_eta * eta;
+  }
+  for (int s = 0; s < scales; ++s) {
+    lv[s].off = total;
+    total += lv[s].n;
+  }
+  float *b1, *b2, *p1, *p2, *pv, *py;
+  WAP_TRY(scratch(c, "ms_b1", n0, &b1));
+  WAP_TRY(scratch(c, "ms_b2", n0, &b2));
+  WAP_TRY(scratch(c, "ms_i1", total, &p1));
+  WAP_TRY(scratch(c, "ms_i2", total, &p2));
+  WAP_TRY(scratch(c, "ms_v", 3 * total, &pv));
+  WAP_TRY(scratch(c, "ms_y", 10 * total, &py));
+  WAP_TRY(blur_impl(c, I1, dims, 1, sigma, b1, st));
+  WAP_TRY(blur_impl(c, I2, dims, 1, sigma, b2, st));
+  for (int s = 0; s < scales; ++s) {
+    WAP_TRY(resize_impl(c, b1, dims, 1, lv[s].d, p1 + lv[s].off, st));
+    WAP_TRY(resize_impl(c, b2, dims, 1, lv[s].d, p2 + lv[s].off, st));
+  }
+  float* v0 = pv + 3 * lv[0].off;
+  if (V_init)
+    WAP_TRY(resize_impl(c, V_init, dims, 3, lv[0].d, v0, st));
+  else
+    WAP_HIP(c, wapk_fill(v0, 3 * lv[0].n, 0.0f, st));
+  WAP_HIP(c, wapk_fill(py + 10 * lv[0].off, 10 * lv[0].n, 0.0f, st));
+  for (int s = 0; s < scales; ++s) {
+    const Lv& L = lv[s];
+    const float aabb[6] = {0.0f, (float)dims[0], 0.0f, (float)dims[1], 0.0f, (float)dims[2]};
+    int roi[6];
+    for (int q = 0; q < 6; ++q) {
+      const int a = q / 2;
+      roi[q] = (int)((float)cut[q] * ((float)L.d[a] / (float)dims[a]));
+    }
+    float* v = pv + 3 * L.off;
+    float* y = py + 10 * L.off;
+    WAP_TRY(flow_level(c, L.d, p1 + L.off, p2 + L.off, v, y, tol, lambda, max_iter, nwarps, aabb,
+                       roi, nullptr, st));
+    if (s + 1 < scales) {
+      const Lv& N = lv[s + 1];
+      WAP_TRY(resize_impl(c, v, L.d, 3, N.d, pv + 3 * N.off, st));
+      WAP_TRY(resize_impl(c, y, L.d, 10, N.d, py + 10 * N.off, st));
+    }
+  }
+  WAP_HIP(c, wapk_copy(V_out, pv + 3 * lv[scales - 1].off, 3 * n0, st));
+  return WAP_OK;
+}
+
+int check_views(WapContext* c, int view0, int nviews) {
+  if (!c->views || view0 < 0 || nviews <= 0 || view0 + nviews > c->nviews)
+    return fail(c, WAP_E_STATE, "view range outside the uploaded view table");
+  return WAP_OK;
+}
+
+int sart_prox_impl(WapContext* c, const WapLoopConfig* L, float* vol, const float* sino,
+                   int first, int last, const float* mask, hipStream_t st) {
+  const size_t nd = (size_t)L->nu * L->nv;
+  const int nw = last - first + 1;
+  float *py, *corr;
+  WAP_TRY(scratch(c, "sart_py", nw * nd, &py));
+  WAP_TRY(scratch(c, "sart_corr", nd, &corr));
+  WAP_HIP(c, wapk_fill(py, nw * nd, 0.0f, st));
+  std::vector<int> order(nw);
+  wap_view_order(first, last, order.data());
+  const int lo[3] = {L->roi[0], L->roi[2], L->roi[4]};
+  const int hi[3] = {L->dims[0] - L->roi[1], L->dims[1] - L->roi[3], L->dims[2] - L->roi[5]};
+  for (int s = 0; s < L->nb_sart; ++s) {
+    for (int q = 0; q < nw; ++q) {
+      const int view = order[q];
+      WAP_HIP(c, wapk_project(vol, 0, L->dims, L->sd, c->views + view, 1, L->nu, L->nv, nullptr,
+                              nullptr, nullptr, sino + (size_t)view * nd,
+                              py + (size_t)(view - first) * nd, corr, L->alpha, st));
+      WAP_HIP(c, wapk_bp_voxel(vol, L->dims, corr, c->views + view, L->nu, L->nv, L->alpha,
+                               L->gmax, lo, hi, 0, st));
+    }
+  }
+  WAP_HIP(c, wapk_roi_apply(vol, mask, nvox(L->dims), st));
+  return WAP_OK;
+}
+
+int warp_project_impl(WapContext* c, const WapLoopConfig* L, float* vols, const float* flows,
+                      const float* b, int frame, int view, float rtime, const float* mask,
+                      hipStream_t st) {
+  const size_t n = nvox(L->dims), nd = (size_t)L->nu * L->nv;
+  float *warped, *corr, *delta, *back;
+  WAP_TRY(scratch(c, "wap_warped", n, &warped));
+  WAP_TRY(scratch(c, "wap_corr", nd, &corr));
+  WAP_TRY(scratch(c, "wap_delta", n, &delta));
+  WAP_TRY(scratch(c, "wap_back", n, &back));
+  const float aabb[6] = {0.0f, (float)L->dims[0], 0.0f, (float)L->dims[1], 0.0f,
+                         (float)L->dims[2]};
+  float* vol = vols + (size_t)frame * n;
+  const int nf = L->nstates;
+  const int fsel = rtime < 0.0f ? std::max(0, frame - 1) : std::min(nf - 1, frame);
+  WAP_HIP(c, wapk_advect(L->dims, aabb, flows + (size_t)fsel * 3 * n, vol, 1, rtime, warped, st));
+  WAP_HIP(c, wapk_project(warped, 0, L->dims, L->sd, c->views + view, 1, L->nu, L->nv, nullptr,
+                          nullptr, nullptr, b, nullptr, corr, L->alpha, st));
+  WAP_HIP(c, wapk_fill(delta, n, 0.0f, st));
+  const int lo[3] = {0, 0, 0};
+  WAP_HIP(c, wapk_bp_voxel(delta, L->dims, corr, c->views + view, L->nu, L->nv, L->alpha, 0.0f,
+                           lo, L->dims, 1, st));
+  WAP_HIP(c, wapk_advect(L->dims, aabb, flows + (size_t)std::max(0, frame - 1) * 3 * n, delta, 1,
+                         -rtime, back, st));
+  WAP_HIP(c, wapk_wap_update(vol, back, L->alpha, L->gmax, mask, n, st));
+  return WAP_OK;
+}
+
+int check_loop(WapContext* c, const WapLoopConfig* L) {
+  if (!L || !dims_ok(L->dims) || L->nu < 5 || L->nv < 5 || L->nstates <= 0 || L->nviews <= 0)
+    return fail(c, WAP_E_ARG, "invalid loop configuration");
+  if (L->nviews > c->nviews) return fail(c, WAP_E_STATE, "upload the view table first");
+  if (!(L->sd > 0.0f)) return fail(c, WAP_E_ARG, "ray step must be positive");
+  if (L->dims[0] < 4 || L->dims[1] < 4 || L->dims[2] < 4)
+    return fail(c, WAP_E_ARG, "every volume extent must be at least 4");
+  return WAP_OK;
+}
+
+}  // namespace
+
+// =================================================================================
+// exported entry points
+// =================================================================================
+extern "C" {
+
+int wap_version(void) { return 1; }
+
+int wap_ctx_create(WapContext** out, int device) {
+  if (!out) return WAP_E_ARG;
+  *out = nullptr;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) return WAP_E_HIP;
+  if (device < 0 || device >= count) return WAP_E_ARG;
+  if (hipSetDevice(device) != hipSuccess) return WAP_E_HIP;
+  WapContext* c = new WapContext();
+  c->device = device;
+  *out = c;
+  return WAP_OK;
+}
+
+int wap_ctx_destroy(WapContext* c) {
+  if (!c) return WAP_E_ARG;
+  (void)hipDeviceSynchronize();
+  for (auto& kv : c->bufs)
+    if (kv.second.ptr) (void)hipFree(kv.second.ptr);
+  for (auto& kv : c->tables) {
+    ResizeTable& t = kv.second;
+    if (t.tstart) (void)hipFree(t.tstart);
+    if (t.tidx) (void)hipFree(t.tidx);
+    if (t.tw) (void)hipFree(t.tw);
+    if (t.pidx) (void)hipFree(t.pidx);
+    if (t.frac) (void)hipFree(t.frac);
+  }
+  if (c->views) (void)hipFree(c->views);
+  delete c;
+  return WAP_OK;
+}
+
+const char* wap_last_error(const WapContext* c) { return c ? c->err.c_str() : "null context"; }
+
+int wap_upload_views(WapContext* c, const WapView* hv, int n) {
+  if (!c || !hv || n <= 0) return fail(c, WAP_E_ARG, "invalid view table");
+  if (n > c->nviews) {
+    if (c->views) (void)hipFree(c->views);
+    c->views = nullptr;
+    c->nviews = 0;
+    if (hipMalloc(&c->views, sizeof(WapView) * n) != hipSuccess)
+      return fail(c, WAP_E_NOMEM, "view table allocation failed");
+  }
+  c->nviews = n;
+  WAP_HIP(c, hipMemcpy(c->views, hv, sizeof(WapView) * n, hipMemcpyHostToDevice));
+  return WAP_OK;
+}
+
+int wap_forward_project(WapContext* c, const float* vol, size_t vol_stride, const int* dims,
+                        float sd, int view0, int nviews, int nu, int nv, float* proj,
+                        float* raylen, float* weight, void* stream) {
+  if (!c || !vol || !proj || !dims_ok(dims) || nu <= 0 || nv <= 0 || !(sd > 0.0f))
+    return fail(c, WAP_E_ARG, "wap_forward_project: invalid arguments");
+  WAP_TRY(check_views(c, view0, nviews));
+  WAP_HIP(c, wapk_project(vol, vol_stride, dims, sd, c->views + view0, nviews, nu, nv, proj,
+                          raylen, weight, nullptr, nullptr, nullptr, 0.0, S(stream)));
+  return WAP_OK;
+}
+
+int wap_project_correct(WapContext* c, const float* vol, const int* dims, float sd, int view,
+                        int nu, int nv, const float* b, float* py, double alpha, float* corr,
+                        float* proj, float* raylen, void* stream) {
+  if (!c || !vol || !b || !corr || !dims_ok(dims) || nu <= 0 || nv <= 0 || !(sd > 0.0f))
+    return fail(c, WAP_E_ARG, "wap_project_correct: invalid arguments");
+  WAP_TRY(check_views(c, view, 1));
+  WAP_HIP(c, wapk_project(vol, 0, dims, sd, c->views + view, 1, nu, nv, proj, raylen, nullptr, b,
+                          py, corr, alpha, S(stream)));
+  return WAP_OK;
+}
+
+int wap_backproject_voxel(WapContext* c, float* vol, const int* dims, const float* corr,
+                          int view, int nu, int nv, double alpha, float gmax, const int* lo,
+                          const int* hi, int mode, void* stream) {
+  if (!c || !vol || !corr || !dims_ok(dims) || !lo || !hi || nu < 5 || nv < 5 ||
+      (mode != 0 && mode != 1))
+    return fail(c, WAP_E_ARG, "wap_backproject_voxel: invalid arguments");
+  for (int a = 0; a < 3; ++a)
+    if (lo[a] < 0 || hi[a] > dims[a] || lo[a] > hi[a])
+      return fail(c, WAP_E_ARG, "wap_backproject_voxel: box outside the volume");
+  WAP_TRY(check_views(c, view, 1));
+  WAP_HIP(c, wapk_bp_voxel(vol, dims, corr, c->views + view, nu, nv, alpha, gmax, lo, hi, mode,
+                           S(stream)));
+  return WAP_OK;
+}
+
+int wap_backproject_ray(WapContext* c, const float* corr, const int* dims, float sd, int view0,
+                        int nviews, int nu, int nv, double scale, float* out, float* wsum,
+                        void* stream) {
+  if (!c || !corr || !out || !dims_ok(dims) || nu <= 0 || nv <= 0 || !(sd > 0.0f))
+    return fail(c, WAP_E_ARG, "wap_backproject_ray: invalid arguments");
+  WAP_TRY(check_views(c, view0, nviews));
+  const size_t n = nvox(dims);
+  double *acc, *wacc = nullptr;
+  WAP_TRY(scratch(c, "scatter_acc", n, &acc));
+  if (wsum) WAP_TRY(scratch(c, "scatter_w", n, &wacc));
+  WAP_HIP(c, wapk_fill_double(acc, n, S(stream)));
+  if (wacc) WAP_HIP(c, wapk_fill_double(wacc, n, S(stream)));
+  WAP_HIP(c, wapk_scatter(corr, dims, sd, c->views + view0, nviews, nu, nv, acc, wacc, S(stream)));
+  WAP_HIP(c, wapk_scatter_finish(acc, wacc, n, scale, out, wsum, S(stream)));
+  return WAP_OK;
+}
+
+int wap_advect(WapContext* c, const int* dims, const float* aabb, const float* vel,
+               const float* src, int nc, float dt, float* out, void* stream) {
+  if (!c || !dims_ok(dims) || !aabb || !vel || !src || !out || nc <= 0)
+    return fail(c, WAP_E_ARG, "wap_advect: invalid arguments");
+  if (dims[0] < 3 || dims[1] < 3 || dims[2] < 3)
+    return fail(c, WAP_E_ARG, "wap_advect: every extent must be at least 3");
+  if (!(aabb[1] > aabb[0] && aabb[3] > aabb[2] && aabb[5] > aabb[4]))
+    return fail(c, WAP_E_ARG, "wap_advect: empty bounding box");
+  WAP_HIP(c, wapk_advect(dims, aabb, vel, src, nc, dt, out, S(stream)));
+  return WAP_OK;
+}
+
+int wap_blur(WapContext* c, const float* src, const int* dims, int nc, float sigma, float* out,
+             void* stream) {
+  if (!c || !src || !out || !dims_ok(dims) || nc <= 0 || sigma < 0.0f)
+    return fail(c, WAP_E_ARG, "wap_blur: invalid arguments");
+  return blur_impl(c, src, dims, nc, sigma, out, S(stream));
+}
+
+int wap_resize(WapContext* c, const float* src, const int* dims, int nc, const int* nd,
+               float* out, void* stream) {
+  if (!c || !src || !out || !dims_ok(dims) || !dims_ok(nd) || nc <= 0)
+    return fail(c, WAP_E_ARG, "wap_resize: invalid arguments");
+  return resize_impl(c, src, dims, nc, nd, out, S(stream));
+}
+
+int wap_flow_solve(WapContext* c, const int* dims, const float* I1, const float* I2, float* V,
+                   float* Y, float tol, float lambda, int max_iter, int nwarps,
+                   const float* aabb, const int* roi, int* iters_run, void* stream) {
+  if (!c || !dims_ok(dims) || !I1 || !I2 || !V || !Y || !aabb || !roi || max_iter < 0 ||
+      nwarps < 0)
+    return fail(c, WAP_E_ARG, "wap_flow_solve: invalid arguments");
+  if (dims[0] < 3 || dims[1] < 3 || dims[2] < 3)
+    return fail(c, WAP_E_ARG, "wap_flow_solve: every extent must be at least 3");
+  for (int q = 0; q < 6; ++q)
+    if (roi[q] < 0) return fail(c, WAP_E_ARG, "wap_flow_solve: negative ROI margin");
+  return flow_level(c, dims, I1, I2, V, Y, tol, lambda, max_iter, nwarps, aabb, roi, iters_run,
+                    S(stream));
+}
+
+int wap_flow_multiscale(WapContext* c, const int* dims, const float* I1, const float* I2,
+                        const float* V_init, float tol, float lambda, int max_iter, int nwarps,
+                        float eta, float sigma, int scales, const int* cut, float* V_out,
+                        void* stream) {
+  if (!c || !dims_ok(dims) || !I1 || !I2 || !V_out || !cut || scales <= 0 || max_iter < 0 ||
+      nwarps < 0 || !(eta > 0.0f && eta <= 1.0f))
+    return fail(c, WAP_E_ARG, "wap_flow_multiscale: invalid arguments");
+  return flow_multiscale_impl(c, dims, I1, I2, V_init, tol, lambda, max_iter, nwarps, eta, sigma,
+                              scales, cut, V_out, S(stream));
+}
+
+int wap_tvtp_step(WapContext* c, int frame, const int* dims, const int* roi6, float* vol,
+                  float* xprev, const float* xbar_f, const float* xbar_next, float* ytv_f,
+                  float* ytp_f, const float* ytp_prev, float cp_sigma, float cp_tau, float tv_w,
+                  float tp_w, float gmax, void* stream) {
+  if (!c || !dims_ok(dims) || !roi6 || !vol || !xprev || !xbar_f || !xbar_next || !ytv_f ||
+      !ytp_f || (frame > 0 && !ytp_prev))
+    return fail(c, WAP_E_ARG, "wap_tvtp_step: invalid arguments");
+  for (int q = 0; q < 6; ++q)
+    if (roi6[q] < 1) return fail(c, WAP_E_ARG, "wap_tvtp_step: ROI margins must be >= 1");
+  WAP_HIP(c, wapk_tvtp(frame, dims, roi6, vol, xprev, xbar_f, xbar_next, ytv_f, ytp_f, ytp_prev,
+                       cp_sigma, cp_tau, tv_w, tp_w, gmax, S(stream)));
+  return WAP_OK;
+}
+
+int wap_xbar_update(WapContext* c, const int* dims, const int* roi6, float* xbar,
+                    const float* vol, const float* xprev, void* stream) {
+  if (!c || !dims_ok(dims) || !roi6 || !xbar || !vol || !xprev)
+    return fail(c, WAP_E_ARG, "wap_xbar_update: invalid arguments");
+  WAP_HIP(c, wapk_xbar(dims, roi6, xbar, vol, xprev, S(stream)));
+  return WAP_OK;
+}
+
+int wap_roi_apply(WapContext* c, float* vol, const float* mask, size_t n, void* stream) {
+  if (!c || !vol || !mask || n == 0) return fail(c, WAP_E_ARG, "wap_roi_apply: invalid arguments");
+  WAP_HIP(c, wapk_roi_apply(vol, mask, n, S(stream)));
+  return WAP_OK;
+}
+
+int wap_sart_prox(WapContext* c, const WapLoopConfig* L, float* vol, const float* sino, int first,
+                  int last, const float* mask, void* stream) {
+  if (!c) return WAP_E_ARG;
+  WAP_TRY(check_loop(c, L));
+  if (!vol || !sino || !mask || first < 0 || last < first || last >= L->nviews)
+    return fail(c, WAP_E_ARG, "wap_sart_prox: invalid arguments");
+  return sart_prox_impl(c, L, vol, sino, first, last, mask, S(stream));
+}
+
+int wap_warp_and_project(WapContext* c, const WapLoopConfig* L, float* vols, const float* flows,
+                         const float* sino, int frame, int view, float rtime, const float* mask,
+                         void* stream) {
+  if (!c) return WAP_E_ARG;
+  WAP_TRY(check_loop(c, L));
+  if (!vols || !flows || !sino || !mask || frame < 0 || frame >= L->nstates || view < 0 ||
+      view >= L->nviews || L->dims[0] < 3 || L->dims[1] < 3 || L->dims[2] < 3)
+    return fail(c, WAP_E_ARG, "wap_warp_and_project: invalid arguments");
+  const size_t nd = (size_t)L->nu * L->nv;
+  return warp_project_impl(c, L, vols, flows, sino + (size_t)view * nd, frame, view, rtime, mask,
+                           S(stream));
+}
+
+int wap_iteration(WapContext* c, const WapLoopConfig* L, float* vols, float* flows,
+                  const float* sino, const int* ws, const int* we, const float* ftime,
+                  const float* mask, void* stream) {
+  if (!c) return WAP_E_ARG;
+  WAP_TRY(check_loop(c, L));
+  if (!vols || !flows || !sino || !ws || !we || !ftime || !mask)
+    return fail(c, WAP_E_ARG, "wap_iteration: invalid arguments");
+  for (int f = 0; f < L->nstates; ++f)
+    if (ws[f] < 0 || we[f] < ws[f] || we[f] >= L->nviews)
+      return fail(c, WAP_E_ARG, "wap_iteration: view window outside the scan");
+  const size_t n = nvox(L->dims), nd = (size_t)L->nu * L->nv;
+  hipStream_t st = S(stream);
+  for (int f = 0; f < L->nstates; ++f)
+    WAP_TRY(sart_prox_impl(c, L, vols + f * n, sino, ws[f], we[f], mask, st));
+  if (L->do_flow) {
+    float* out;
+    WAP_TRY(scratch(c, "iter_flow", 3 * n, &out));
+    for (int f = 0; f + 1 < L->nstates; ++f) {
+      WAP_TRY(flow_multiscale_impl(c, L->dims, vols + f * n, vols + (f + 1) * n,
+                                   flows + f * 3 * n, L->flow_tol, L->flow_lambda,
+                                   L->flow_iters, L->flow_warps, L->flow_eta, L->flow_sigma,
+                                   L->flow_scales, L->cut, out, st));
+      WAP_HIP(c, wapk_copy(flows + f * 3 * n, out, 3 * n, st));
+    }
+  }
+  if (L->do_warp) {
+    for (int f = 0; f < L->nstates; ++f)
+      for (int ik = ws[f]; ik <= we[f]; ++ik) {
+        float td = ((float)ik - ftime[f]) / L->timeunit;
+        td = std::min(std::max(-1.0f, td), 1.0f);
+        WAP_TRY(warp_project_impl(c, L, vols, flows, sino + ik * nd, f, ik, td, mask, st));
+      }
+  }
+  return WAP_OK;
+}
+
+}  // extern "C"

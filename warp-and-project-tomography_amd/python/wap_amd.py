@@ -1,0 +1,288 @@
+"""Python host mirror of the wap C ABI (include/wap.h) over ctypes.
+
+Device memory and streams come from PyTorch-ROCm (tensors on ``cuda:N`` and
+``torch.cuda.current_stream()``); all compute runs in libwap.so's gfx950 kernels.
+There is no CPU fallback: if the library is missing, ``load()`` raises.
+
+Arrays are planar x-fastest: a W x H x D x C field is a tensor of shape (C, D, H, W)
+(or (D, H, W) for one plane); detector stacks are (views, Nv, Nu).
+"""
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.normpath(os.path.join(HERE, "..", "libwap.so"))
+
+WAP_OK = 0
+
+
+class WapScanGeometry(C.Structure):
+    _fields_ = [("vol_dims", C.c_int * 3), ("vol_spacing", C.c_float),
+                ("det_u", C.c_int), ("det_v", C.c_int), ("det_pitch", C.c_double),
+                ("sid", C.c_float), ("sdd", C.c_float), ("translate", C.c_float * 3),
+                ("counterclockwise", C.c_int), ("parallel", C.c_int), ("pad_", C.c_int)]
+
+
+class WapView(C.Structure):
+    _fields_ = [("p2m", C.c_float * 16), ("m2v", C.c_float * 16), ("mvv2m", C.c_float * 16),
+                ("eye", C.c_float * 4), ("dir", C.c_float * 4), ("half_w", C.c_double),
+                ("half_h", C.c_double), ("ds", C.c_double), ("sdd", C.c_float),
+                ("parallel", C.c_int)]
+
+
+class WapLoopConfig(C.Structure):
+    _fields_ = [("dims", C.c_int * 3), ("nstates", C.c_int), ("nviews", C.c_int),
+                ("nu", C.c_int), ("nv", C.c_int), ("sd", C.c_float), ("alpha", C.c_double),
+                ("gmax", C.c_float), ("nb_sart", C.c_int), ("roi", C.c_int * 8),
+                ("cut", C.c_int * 8), ("flow_tol", C.c_float), ("flow_lambda", C.c_float),
+                ("flow_eta", C.c_float), ("flow_sigma", C.c_float), ("flow_iters", C.c_int),
+                ("flow_warps", C.c_int), ("flow_scales", C.c_int), ("do_flow", C.c_int),
+                ("do_warp", C.c_int), ("timeunit", C.c_float)]
+
+
+assert C.sizeof(WapScanGeometry) == 64 and C.sizeof(WapView) == 256
+
+# name -> argtypes (restype int unless noted)
+_vp = C.c_void_p
+_ip = C.POINTER(C.c_int)
+_fp = C.POINTER(C.c_float)
+SIGNATURES = {
+    "wap_version": [],
+    "wap_ctx_create": [C.POINTER(_vp), C.c_int],
+    "wap_ctx_destroy": [_vp],
+    "wap_last_error": [_vp],
+    "wap_view_geometry": [C.POINTER(WapScanGeometry), C.c_double, C.POINTER(WapView)],
+    "wap_view_order": [C.c_int, C.c_int, _ip],
+    "wap_frame_windows": [C.c_int, C.c_int, C.c_float, _fp, _ip, _ip],
+    "wap_roi_mask": [_ip, _ip, _fp],
+    "wap_upload_views": [_vp, C.POINTER(WapView), C.c_int],
+    "wap_forward_project": [_vp, _vp, C.c_size_t, _ip, C.c_float, C.c_int, C.c_int, C.c_int,
+                            C.c_int, _vp, _vp, _vp, _vp],
+    "wap_project_correct": [_vp, _vp, _ip, C.c_float, C.c_int, C.c_int, C.c_int, _vp, _vp,
+                            C.c_double, _vp, _vp, _vp, _vp],
+    "wap_backproject_voxel": [_vp, _vp, _ip, _vp, C.c_int, C.c_int, C.c_int, C.c_double,
+                              C.c_float, _ip, _ip, C.c_int, _vp],
+    "wap_backproject_ray": [_vp, _vp, _ip, C.c_float, C.c_int, C.c_int, C.c_int, C.c_int,
+                            C.c_double, _vp, _vp, _vp],
+    "wap_advect": [_vp, _ip, _fp, _vp, _vp, C.c_int, C.c_float, _vp, _vp],
+    "wap_blur": [_vp, _vp, _ip, C.c_int, C.c_float, _vp, _vp],
+    "wap_resize": [_vp, _vp, _ip, C.c_int, _ip, _vp, _vp],
+    "wap_flow_solve": [_vp, _ip, _vp, _vp, _vp, _vp, C.c_float, C.c_float, C.c_int, C.c_int,
+                       _fp, _ip, _ip, _vp],
+    "wap_flow_multiscale": [_vp, _ip, _vp, _vp, _vp, C.c_float, C.c_float, C.c_int, C.c_int,
+                            C.c_float, C.c_float, C.c_int, _ip, _vp, _vp],
+    "wap_tvtp_step": [_vp, C.c_int, _ip, _ip, _vp, _vp, _vp, _vp, _vp, _vp, _vp, C.c_float,
+                      C.c_float, C.c_float, C.c_float, C.c_float, _vp],
+    "wap_xbar_update": [_vp, _ip, _ip, _vp, _vp, _vp, _vp],
+    "wap_roi_apply": [_vp, _vp, _vp, C.c_size_t, _vp],
+    "wap_sart_prox": [_vp, C.POINTER(WapLoopConfig), _vp, _vp, C.c_int, C.c_int, _vp, _vp],
+    "wap_warp_and_project": [_vp, C.POINTER(WapLoopConfig), _vp, _vp, _vp, C.c_int, C.c_int,
+                             C.c_float, _vp, _vp],
+    "wap_iteration": [_vp, C.POINTER(WapLoopConfig), _vp, _vp, _vp, _ip, _ip, _fp, _vp, _vp],
+}
+
+_lib = None
+
+
+def load(path=LIB_PATH):
+    """Load libwap.so once; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(f"libwap.so not built ({path}); run __graft_entry__.build() or "
+                           "make -C warp-and-project-tomography_amd/csrc")
+    lib = C.CDLL(path)
+    for name, args in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = C.c_char_p if name == "wap_last_error" else C.c_int
+    _lib = lib
+    return lib
+
+
+def ints(*v):
+    return (C.c_int * len(v))(*[int(x) for x in v])
+
+
+def floats(*v):
+    return (C.c_float * len(v))(*[float(x) for x in v])
+
+
+def make_geometry(dims_whd, nu, nv, ds=1.0, sid=400.0, sdd=600.0, spacing=1.0, parallel=False,
+                  ccw=True, translate=(0.0, 0.0, 0.0)):
+    g = WapScanGeometry()
+    g.vol_dims[:] = [int(x) for x in dims_whd]
+    g.vol_spacing = spacing
+    g.det_u, g.det_v = int(nu), int(nv)
+    g.det_pitch = ds
+    g.sid, g.sdd = sid, sdd
+    g.translate[:] = list(translate)
+    g.counterclockwise = 1 if ccw else 0
+    g.parallel = 1 if parallel else 0
+    return g
+
+
+def view_geometry(geom, angle_deg):
+    lib = load()
+    v = WapView()
+    r = lib.wap_view_geometry(C.byref(geom), float(angle_deg), C.byref(v))
+    if r != WAP_OK:
+        raise ValueError(f"wap_view_geometry failed ({r})")
+    return v
+
+
+def views(geom, angles):
+    arr = (WapView * len(angles))()
+    lib = load()
+    for i, a in enumerate(angles):
+        r = lib.wap_view_geometry(C.byref(geom), float(a), C.byref(arr[i]))
+        if r != WAP_OK:
+            raise ValueError(f"wap_view_geometry failed ({r})")
+    return arr
+
+
+def _dims(t):
+    d, h, w = t.shape[-3:]
+    return ints(w, h, d)
+
+
+def _ptr(t):
+    if t is None:
+        return None
+    if not t.is_cuda:
+        raise ValueError("wap kernels take device tensors")
+    if not t.is_contiguous():
+        raise ValueError("wap kernels take contiguous tensors")
+    return C.c_void_p(t.data_ptr())
+
+
+class Context:
+    """A wap context bound to one device; every call is queued on the current stream."""
+
+    def __init__(self, device=0):
+        import torch  # noqa: F401  (initialises the HIP runtime on this device)
+        self.lib = load()
+        self.device = device
+        h = C.c_void_p()
+        r = self.lib.wap_ctx_create(C.byref(h), device)
+        if r != WAP_OK:
+            raise RuntimeError(f"wap_ctx_create failed ({r})")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            self.lib.wap_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _stream(self):
+        import torch
+        return C.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
+
+    def _check(self, r, what):
+        if r != WAP_OK:
+            msg = self.lib.wap_last_error(self.h)
+            raise RuntimeError(f"{what} failed ({r}): {msg.decode() if msg else ''}")
+
+    # -- geometry -------------------------------------------------------------------
+    def upload_views(self, view_array):
+        self._check(self.lib.wap_upload_views(self.h, view_array, len(view_array)),
+                    "wap_upload_views")
+
+    # -- projector ------------------------------------------------------------------
+    def forward_project(self, vol, sd, view0, nviews, nu, nv, proj, raylen=None, weight=None,
+                        vol_stride=0):
+        self._check(self.lib.wap_forward_project(
+            self.h, _ptr(vol), vol_stride, _dims(vol), sd, view0, nviews, nu, nv, _ptr(proj),
+            _ptr(raylen), _ptr(weight), self._stream()), "wap_forward_project")
+
+    def project_correct(self, vol, sd, view, nu, nv, b, py, alpha, corr, proj=None,
+                        raylen=None):
+        self._check(self.lib.wap_project_correct(
+            self.h, _ptr(vol), _dims(vol), sd, view, nu, nv, _ptr(b), _ptr(py), alpha,
+            _ptr(corr), _ptr(proj), _ptr(raylen), self._stream()), "wap_project_correct")
+
+    def backproject_voxel(self, vol, corr, view, alpha, gmax, lo, hi, mode):
+        nv, nu = corr.shape[-2:]
+        self._check(self.lib.wap_backproject_voxel(
+            self.h, _ptr(vol), _dims(vol), _ptr(corr), view, nu, nv, alpha, gmax, ints(*lo),
+            ints(*hi), mode, self._stream()), "wap_backproject_voxel")
+
+    def backproject_ray(self, corr, dhw, sd, view0, nviews, scale, out, wsum=None):
+        nv, nu = corr.shape[-2:]
+        self._check(self.lib.wap_backproject_ray(
+            self.h, _ptr(corr), ints(dhw[2], dhw[1], dhw[0]), sd, view0, nviews, nu, nv, scale,
+            _ptr(out), _ptr(wsum), self._stream()), "wap_backproject_ray")
+
+    # -- advection / image ops --------------------------------------------------------
+    def advect(self, vel, src, dt, out, aabb=None):
+        d, h, w = src.shape[-3:]
+        nc = 1 if src.dim() == 3 else src.shape[0]
+        if aabb is None:
+            aabb = (0, w, 0, h, 0, d)
+        self._check(self.lib.wap_advect(self.h, ints(w, h, d), floats(*aabb), _ptr(vel),
+                                        _ptr(src), nc, dt, _ptr(out), self._stream()),
+                    "wap_advect")
+
+    def blur(self, src, sigma, out):
+        nc = 1 if src.dim() == 3 else src.shape[0]
+        self._check(self.lib.wap_blur(self.h, _ptr(src), _dims(src), nc, sigma, _ptr(out),
+                                      self._stream()), "wap_blur")
+
+    def resize(self, src, out):
+        nc = 1 if src.dim() == 3 else src.shape[0]
+        self._check(self.lib.wap_resize(self.h, _ptr(src), _dims(src), nc, _dims(out),
+                                        _ptr(out), self._stream()), "wap_resize")
+
+    # -- flow ----------------------------------------------------------------------
+    def flow_solve(self, I1, I2, V, Y, tol, lam, max_iter, nwarps, aabb, roi, count=False):
+        it = C.c_int(0)
+        self._check(self.lib.wap_flow_solve(
+            self.h, _dims(I1), _ptr(I1), _ptr(I2), _ptr(V), _ptr(Y), tol, lam, max_iter,
+            nwarps, floats(*aabb), ints(*roi), C.byref(it) if count else None,
+            self._stream()), "wap_flow_solve")
+        return it.value
+
+    def flow_multiscale(self, I1, I2, V_init, tol, lam, max_iter, nwarps, eta, sigma, scales,
+                        cut, V_out):
+        self._check(self.lib.wap_flow_multiscale(
+            self.h, _dims(I1), _ptr(I1), _ptr(I2), _ptr(V_init), tol, lam, max_iter, nwarps,
+            eta, sigma, scales, ints(*cut), _ptr(V_out), self._stream()),
+            "wap_flow_multiscale")
+
+    def tvtp_step(self, frame, roi6, vol, xprev, xbar_f, xbar_next, ytv_f, ytp_f, ytp_prev,
+                  cp_sigma, cp_tau, tv_w, tp_w, gmax):
+        self._check(self.lib.wap_tvtp_step(
+            self.h, frame, _dims(vol), ints(*roi6), _ptr(vol), _ptr(xprev), _ptr(xbar_f),
+            _ptr(xbar_next), _ptr(ytv_f), _ptr(ytp_f), _ptr(ytp_prev), cp_sigma, cp_tau, tv_w,
+            tp_w, gmax, self._stream()), "wap_tvtp_step")
+
+    def xbar_update(self, roi6, xbar, vol, xprev):
+        self._check(self.lib.wap_xbar_update(self.h, _dims(vol), ints(*roi6), _ptr(xbar),
+                                             _ptr(vol), _ptr(xprev), self._stream()),
+                    "wap_xbar_update")
+
+    def roi_apply(self, vol, mask):
+        self._check(self.lib.wap_roi_apply(self.h, _ptr(vol), _ptr(mask), vol.numel(),
+                                           self._stream()), "wap_roi_apply")
+
+    # -- loop ----------------------------------------------------------------------
+    def sart_prox(self, cfg, vol, sino, first, last, mask):
+        self._check(self.lib.wap_sart_prox(self.h, C.byref(cfg), _ptr(vol), _ptr(sino), first,
+                                           last, _ptr(mask), self._stream()), "wap_sart_prox")
+
+    def warp_and_project(self, cfg, vols, flows, sino, frame, view, rtime, mask):
+        self._check(self.lib.wap_warp_and_project(
+            self.h, C.byref(cfg), _ptr(vols), _ptr(flows), _ptr(sino), frame, view, rtime,
+            _ptr(mask), self._stream()), "wap_warp_and_project")
+
+    def iteration(self, cfg, vols, flows, sino, ws, we, ftime, mask):
+        self._check(self.lib.wap_iteration(
+            self.h, C.byref(cfg), _ptr(vols), _ptr(flows), _ptr(sino), ints(*ws), ints(*we),
+            floats(*ftime), _ptr(mask), self._stream()), "wap_iteration")
