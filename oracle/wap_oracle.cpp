@@ -26,6 +26,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <limits>
 #include <vector>
 
 namespace {
@@ -264,8 +265,16 @@ inline Ray<R> produce_ray(const WapView& v, const int* dims, int i, int j) {
                    (R)(float)(dims[2] - 1)};
   R tmin[3], tmax[3];
   for (int a = 0; a < 3; ++a) {
-    const R t1 = (cb[2 * a] - r.zero[a]) / r.unit[a];
-    const R t2 = (cb[2 * a + 1] - r.zero[a]) / r.unit[a];
+    R t1 = (cb[2 * a] - r.zero[a]) / r.unit[a];
+    R t2 = (cb[2 * a + 1] - r.zero[a]) / r.unit[a];
+    // Resolved defect: a ray lying in a face plane (zero direction component, origin on
+    // the plane) gives 0/0 here, and the reference's min/max macros then drop this
+    // axis's exit bound, leaving tfar ~1e7 and a march of millions of samples. Such a
+    // ray is inside that slab for every t.
+    if (t1 != t1 || t2 != t2) {
+      t1 = -std::numeric_limits<R>::infinity();
+      t2 = std::numeric_limits<R>::infinity();
+    }
     tmin[a] = mmin(t1, t2);
     tmax[a] = mmax(t1, t2);
   }
@@ -303,14 +312,27 @@ inline void cast_ray(const Ray<R>& r, const int* dims, R sd, F&& visit) {
       tDelta[a] = (R)(float)bs / r.unit[a];
     }
   }
-  while (t1 < tfar) {
+  // Resolved defect: a DDA plane far behind the entry (an axis with a tiny negative
+  // direction component) makes the reference restart the next segment at a t so
+  // negative that t += sd no longer advances and the march never ends. A restart more
+  // than the volume's perimeter behind the entry is taken from the entry instead;
+  // rays that stay within that distance keep the reference's sample set exactly.
+  const R span = (R)(float)(dims[0] + dims[1] + dims[2]);
+  // Hard bound on visits, far above any ray's legitimate count (a chord is at most
+  // the perimeter; restarts add at most three more), so every march ends.
+  const long cap = (long)(8.0f * (float)(dims[0] + dims[1] + dims[2]) / (float)sd) + 16;
+  long visits = 1;
+  int segments = 0;  // a ray crosses at most three 0.5 planes before its exit
+  while (t1 < tfar && segments++ < 16) {
     R tb = mmin(mmin(tMax[0], tMax[1]), tMax[2]);
     if (tb > tfar) tb = tfar;
+    if (tnear - t1 > span) t1 = tnear;
     R t = std::ceil((t1 - tnear) / sd) * sd + tnear;
     R pos[3];
     for (int a = 0; a < 3; ++a) pos[a] = r.zero[a] + r.unit[a] * t;
-    while (t < tb) {
+    while (t < tb && visits < cap) {
       visit(pos);
+      ++visits;
       t += sd;
       for (int a = 0; a < 3; ++a) pos[a] += step[a];
     }

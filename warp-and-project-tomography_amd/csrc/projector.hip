@@ -55,8 +55,14 @@ WAP_DEV RaySeg produce_ray(const WapView& v, int W, int H, int D, int i, int j) 
   float tmin[3], tmax[3];
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
-    const float t1 = (0.0f - zz[a]) / uu[a];
-    const float t2 = (hi[a] - zz[a]) / uu[a];
+    float t1 = (0.0f - zz[a]) / uu[a];
+    float t2 = (hi[a] - zz[a]) / uu[a];
+    // a ray lying in a face plane gives 0/0: it is inside that slab for every t (the
+    // reference's min/max would otherwise drop this axis's exit bound)
+    if (t1 != t1 || t2 != t2) {
+      t1 = -INFINITY;
+      t2 = INFINITY;
+    }
     tmin[a] = mmin(t1, t2);
     tmax[a] = mmax(t1, t2);
     r.z[a] = zz[a];
@@ -99,13 +105,23 @@ WAP_DEV void march(const RaySeg& r, int W, int H, int D, float sd, F&& visit) {
       tDel[a] = (float)bs[a] / r.u[a];
     }
   }
-  while (t1 < tf) {
+  // A segment restart more than the volume's perimeter behind the entry (a far-behind
+  // DDA plane of a nearly axis-parallel ray) is taken from the entry: restarting from
+  // there, as the reference would, reaches t where t += sd stops advancing.
+  const float span = (float)(W + H + D);
+  // hard bound on visits, far above any ray's legitimate count, so every march ends
+  const int cap = (int)(8.0f * (float)(W + H + D) / sd) + 16;
+  int visits = 1;
+  int segments = 0;  // a ray crosses at most three 0.5 planes before its exit
+  while (t1 < tf && segments++ < 16) {
     float tb = mmin(mmin(tMax[0], tMax[1]), tMax[2]);
     if (tb > tf) tb = tf;
+    if (tn - t1 > span) t1 = tn;
     float t = ceilf((t1 - tn) / sd) * sd + tn;
     float px = r.z[0] + r.u[0] * t, py = r.z[1] + r.u[1] * t, pz = r.z[2] + r.u[2] * t;
-    while (t < tb) {
+    while (t < tb && visits < cap) {
       visit(px, py, pz);
+      ++visits;
       t += sd;
       px += step[0];
       py += step[1];
