@@ -1,0 +1,280 @@
+"""Benchmark: warp-and-project outer iterations per second on one MI355X.
+
+Workload (the plain FP32 GPU configuration): 256^3 fp32 volume, 192 deformation states
+with one view each, a foam phantom (unit-density cube with 400 ellipsoid voids)
+compressed over time (y scaled by 1 - 0.15*t/191 about the centre plus a smooth random
+displacement of at most 2 voxels), cone beam with source distance 4L and source-detector
+distance 6L (detector 2L beyond the volume centre), 192 golden-ratio views over 360
+degrees on a 384x384 detector, 1% Gaussian noise. One step = one WaP outer iteration:
+10 SART sweeps per state, the multiscale L1 flow of all 191 adjacent pairs (5 scales,
+3 warps, 50 primal-dual steps per warp, tol 0.008, lambda 0.05), and one
+warp-and-project pass over all views.
+
+Prints one JSON line with the metric, the projector throughput (GRays/s over a full
+192-view sweep), the roofline of the dominant kernel (the flow primal-dual iteration,
+timed live with events on the stream it runs on) and the oracle's CPU time for the same
+work extrapolated from a bounded sample of kernel units.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "warp-and-project-tomography_amd", "python"))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=1)
+    p.add_argument("--warmup", type=int, default=1)
+    p.add_argument("--size", type=int, default=256)
+    p.add_argument("--states", type=int, default=192)
+    p.add_argument("--det", type=int, default=384)
+    p.add_argument("--sart", type=int, default=10)
+    p.add_argument("--flow-iters", type=int, default=50)
+    p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--cpu-sample-seconds", type=float, default=20.0)
+    return p.parse_args()
+
+
+def make_workload(ctx, W, L, S, NDET, torch, dev):
+    import numpy as np
+
+    import phantoms
+    rng = phantoms.SplitMix64(phantoms.SEED_BASE + 2)
+    base = torch.from_numpy(phantoms.foam_phantom(L, 400, rng)).to(dev)
+    noise = torch.from_numpy(phantoms.smooth_field((32, 32, 32), 8, 2, 1.0, rng)).to(dev)
+    # states: sample the base phantom at x' = A_t^{-1}(x - c) + c + n(x) * 2 t/(S-1)
+    lin = torch.linspace(-1.0, 1.0, L, device=dev)
+    gz, gy, gx = torch.meshgrid(lin, lin, lin, indexing="ij")
+    nf = torch.nn.functional.interpolate(noise[None], size=(L, L, L), mode="trilinear",
+                                         align_corners=True)[0]
+    vols = torch.empty((S, L, L, L), device=dev)
+    for s in range(S):
+        frac = s / max(S - 1, 1)
+        sy = 1.0 - 0.15 * frac
+        amp = 2.0 * frac * 2.0 / (L - 1)  # 2 voxels in normalised units
+        grid = torch.stack([gx + amp * nf[0], gy / sy + amp * nf[1], gz + amp * nf[2]], -1)
+        vols[s] = torch.nn.functional.grid_sample(base[None, None], grid[None], mode="bilinear",
+                                                  padding_mode="zeros", align_corners=True)[0, 0]
+    del gz, gy, gx, nf
+    geo = W.make_geometry((L, L, L), NDET, NDET, ds=1.0, sid=4.0 * L, sdd=6.0 * L)
+    angles = phantoms.golden_angles(S, 360.0)
+    views = W.views(geo, angles)
+    ctx.upload_views(views)
+    sd = float(np.float32(math.sqrt(3.0) / 2.0))
+    sino = torch.zeros((S, NDET, NDET), device=dev)
+    ctx.forward_project(vols, sd, 0, S, NDET, NDET, sino, vol_stride=L * L * L)
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(phantoms.SEED_BASE + 2)
+    sino += torch.randn(sino.shape, generator=gen, device=dev) * (0.01 * float(sino.max()))
+    return vols, views, angles, sino, sd
+
+
+def roofline_pd(ctx, torch, dev, L):
+    """Average duration of one flow primal-dual iteration at full resolution, from two
+    solves that differ only in the iteration count, timed with events on the stream the
+    kernels run on (torch's current stream)."""
+    I1 = torch.rand((L, L, L), device=dev)
+    I2 = torch.rand((L, L, L), device=dev)
+    V = torch.zeros((3, L, L, L), device=dev)
+    Y = torch.zeros((10, L, L, L), device=dev)
+    aabb = [0, L, 0, L, 0, L]
+    roi = [8] * 6
+
+    def run(iters):
+        V.zero_()
+        Y.zero_()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        ctx.flow_solve(I1, I2, V, Y, 0.0, 0.05, iters, 1, aabb, roi)
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1)
+
+    run(5)
+    t_lo = min(run(10) for _ in range(2))
+    t_hi = min(run(60) for _ in range(2))
+    per_ms = (t_hi - t_lo) / 50.0
+    n = L * L * L
+    # compulsory bytes per iteration: read v (3), y (10), u (3), ut (1); write v (3), y (10)
+    algo = 30 * 4 * n
+    ach = algo / (per_ms * 1e-3) / 1e9
+    return {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+            "kernel": "flow primal-dual iteration (k_flow_primal + k_flow_dual)",
+            "ms_per_launch": round(per_ms, 4), "bytes_per_launch": algo}
+
+
+def projector_rate(ctx, torch, vols, sd, S, NDET, L):
+    proj = torch.empty((S, NDET, NDET), device=vols.device)
+    ctx.forward_project(vols, sd, 0, S, NDET, NDET, proj, vol_stride=L * L * L)
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    reps = 3
+    e0.record()
+    for _ in range(reps):
+        ctx.forward_project(vols, sd, 0, S, NDET, NDET, proj, vol_stride=L * L * L)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    rays = S * NDET * NDET
+    algo = S * (L ** 3 * 4 + NDET * NDET * 4)
+    gbs = algo / (ms * 1e-3) / 1e9
+    return {"gr_per_s": round(rays / (ms * 1e-3) / 1e9, 4), "ms_per_sweep": round(ms, 3),
+            "hbm_gbs_algorithmic": round(gbs, 1), "hbm_frac": round(gbs / HBM_PEAK_GBS, 4)}
+
+
+def cpu_baseline(L, S, NDET, sart, flow_iters, budget_s):
+    """The oracle (our OpenMP CPU restatement) timed on this host on single kernel units
+    at full size, extrapolated to one outer iteration with the same unit counts."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    import phantoms
+    O = pyoracle.Oracle()
+    rng = np.random.default_rng(0)
+    vol = rng.random((L, L, L), dtype=np.float32)
+    geo = pyoracle.make_geometry((L, L, L), NDET, NDET, sid=4.0 * L, sdd=6.0 * L)
+    v = O.view(geo, 37.0)
+    sd = np.float32(math.sqrt(3.0) / 2.0)
+    t = time.time()
+    ax, rl, _ = O.forward_project(vol, v, NDET, NDET, sd)
+    t_fp = time.time() - t
+    corr = rng.standard_normal((NDET, NDET)).astype(np.float32) * 0.01
+    t = time.time()
+    O.backproject_voxel(vol, corr, v, 0.1, 10.0, (8, 8, 8), (L - 8,) * 3, 0)
+    t_bp = time.time() - t
+    vel = (rng.standard_normal((3, L, L, L)) * 0.5).astype(np.float32)
+    t = time.time()
+    O.advect(vel, vol, -1.0)
+    t_adv = time.time() - t
+    z3 = np.zeros((3, L, L, L), np.float32)
+    z1 = np.zeros((L, L, L), np.float32)
+    aabb = [0, L, 0, L, 0, L]
+    t = time.time()
+    O.flow_single(vol, vol, z3, z3, z3, z3, z1, 0.0, 0.05, 0, 1, aabb, [8] * 6)
+    t0 = time.time() - t
+    t = time.time()
+    O.flow_single(vol, vol, z3, z3, z3, z3, z1, 0.0, 0.05, 2, 1, aabb, [8] * 6)
+    t_pd = max((time.time() - t - t0) / 2.0, 1e-6)
+    prep = max(t0 - t_pd, 0.0)  # one warp's advect/gradient/setup (max_iter 0 runs one step)
+    pyr = sum(0.125 ** k for k in range(5))  # 5-level pyramid, eta = 0.5
+    sart_s = S * sart * (t_fp + t_bp)
+    flow_s = (S - 1) * pyr * 3 * (prep + (flow_iters + 1) * t_pd)
+    warp_s = S * (2 * t_adv + t_fp + t_bp)
+    per_iter = sart_s + flow_s + warp_s
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": round(1.0 / per_iter, 6), "unit": "iters/s", "cores": cores,
+            "kind": "port",
+            "sample": (f"oracle kernel units at {L}^3/{NDET}^2 (1 view projection {t_fp:.2f}s, "
+                       f"1 voxel backprojection {t_bp:.2f}s, 1 advection {t_adv:.2f}s, "
+                       f"1 primal-dual iteration {t_pd:.2f}s, warp setup {prep:.2f}s) "
+                       f"extrapolated to one outer iteration ({per_iter:.0f}s)")}
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1",
+                          "n_gpus": a.gpus}))
+        sys.exit(2)
+    import numpy as np
+    import torch
+
+    import wap_amd as W
+    dev = "cuda:0"
+    L, S, NDET = a.size, a.states, a.det
+    ctx = W.Context(0)
+    print(f"[bench] generating {L}^3 x {S} states, {NDET}^2 detector", file=sys.stderr,
+          flush=True)
+    vols_truth, views, angles, sino, sd = make_workload(ctx, W, L, S, NDET, torch, dev)
+    proj_stats = projector_rate(ctx, torch, vols_truth, sd, S, NDET, L)
+    del vols_truth
+    torch.cuda.empty_cache()
+    cfg = W.WapLoopConfig()
+    cfg.dims[:] = [L, L, L]
+    cfg.nstates, cfg.nviews, cfg.nu, cfg.nv = S, S, NDET, NDET
+    cfg.sd, cfg.alpha = sd, 0.1
+    cfg.gmax = float(sino[0].max())
+    cfg.nb_sart = a.sart
+    roi = max(1, round(8 * L / 256))
+    cfg.roi[:] = [roi] * 8
+    cfg.cut[:] = [roi] * 8
+    cfg.flow_tol, cfg.flow_lambda, cfg.flow_eta, cfg.flow_sigma = 0.008, 0.05, 0.5, 0.8
+    cfg.flow_iters, cfg.flow_warps, cfg.flow_scales = a.flow_iters, 3, 5
+    cfg.do_flow, cfg.do_warp, cfg.timeunit = 1, 1, 1.0
+    ws = list(range(S))
+    we = list(range(S))
+    ft = [float(i) for i in range(S)]
+    mask = torch.from_numpy(_roi_mask(W, L, roi)).to(dev)
+    vols = torch.zeros((S, L, L, L), device=dev)
+    flows = torch.zeros((S, 3, L, L, L), device=dev)
+    for i in range(a.warmup):
+        ctx.iteration(cfg, vols, flows, sino, ws, we, ft, mask)
+        torch.cuda.synchronize()
+        print(f"[bench] warmup step {i + 1} done", file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(a.steps):
+        ctx.iteration(cfg, vols, flows, sino, ws, we, ft, mask)
+        torch.cuda.synchronize()
+        print(f"[bench] timed step {i + 1} done", file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    assert torch.isfinite(vols).all() and float(vols.max()) > 0.0
+    del flows, vols
+    torch.cuda.empty_cache()
+    roof = roofline_pd(ctx, torch, dev, L)
+    cpu = None
+    if not a.no_cpu_baseline:
+        print("[bench] timing the CPU oracle sample", file=sys.stderr, flush=True)
+        cpu = cpu_baseline(L, S, NDET, a.sart, a.flow_iters, a.cpu_sample_seconds)
+    ms = dt * 1000.0 / a.steps
+    out = {
+        "metric": "WaP outer iters/s at 256^3 x 192 views",
+        "value": round(a.steps / dt, 6),
+        "unit": "iters/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(ms, 1),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": (f"plain FP32: {L}^3 volume, {S} states x 1 view, foam "
+                                f"phantom, cone beam SID=4L SDD=6L, {NDET}^2 detector, "
+                                f"{a.sart} SART sweeps, 5-scale L1 flow with 3 warps x "
+                                f"{a.flow_iters} PD steps, warp-and-project pass"),
+                   "volume": L, "states": S, "views": S, "detector": NDET,
+                   "sart_sweeps": a.sart, "flow_pd_steps": a.flow_iters},
+        "projector": proj_stats,
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+    ctx.close()
+
+
+def _roi_mask(W, L, roi):
+    import ctypes as C
+
+    import numpy as np
+    m = np.zeros((L, L, L), np.float32)
+    W.load().wap_roi_mask(W.ints(L, L, L), W.ints(*([roi] * 8)),
+                          m.ctypes.data_as(C.POINTER(C.c_float)))
+    return m
+
+
+if __name__ == "__main__":
+    main()
