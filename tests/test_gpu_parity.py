@@ -53,11 +53,13 @@ def test_forward_project(wctx, oracle, W, parallel, shape, nu, nv):
         np.testing.assert_array_equal(Wt[k], c)
         np.testing.assert_array_equal(R[k], b)
         np.testing.assert_array_equal(P[k], a)
-        if parallel and angles[k] % 90.0 == 0.0:
-            continue  # rays on the volume's face planes: FP32/FP64 disagree on hit or miss
-        f64 = oracle.forward_project(vol.astype(np.float64), views[k], nu, nv, float(SD),
-                                     f64=True)[0]
-        assert rel_l2(P[k], f64) < 2e-3  # float vs double sample sets can differ by one
+        f64, _, w64 = oracle.forward_project(vol.astype(np.float64), views[k], nu, nv,
+                                             float(SD), f64=True)
+        # rays lying exactly on a face plane (grid-aligned parallel pixels) are hits in
+        # one precision and misses in the other; compare the rays both classify alike
+        same = (Wt[k] > 0) == (w64 > 0)
+        assert same.mean() > 0.9
+        assert rel_l2(P[k][same], f64[same]) < 2e-3  # sample sets can differ by one
     assert (Wt > 0).any()
 
 
