@@ -109,7 +109,7 @@ def roofline_pd(ctx, torch, dev, L):
     ach = algo / (per_ms * 1e-3) / 1e9
     return {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
-            "kernel": "flow primal-dual iteration (k_flow_primal + k_flow_dual)",
+            "kernel": "k_flow_pd (one fused primal-dual iteration)",
             "ms_per_launch": round(per_ms, 4), "bytes_per_launch": algo}
 
 

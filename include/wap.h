@@ -38,6 +38,12 @@ int wap_ctx_destroy(WapContext* ctx);
 const char* wap_last_error(const WapContext* ctx);
 int wap_version(void);
 
+/* Tuning knobs (no reference counterpart; results are identical for every value):
+ *   "flow_variant": 0 = two-kernel primal/dual iteration, 1..5 = fused z-marching
+ *   iteration with tiles (64,4,2) (64,4,4) (64,4,8) (32,8,4) (64,4,16). */
+int wap_set_tuning(WapContext* ctx, const char* key, int value);
+int wap_get_tuning(const WapContext* ctx, const char* key, int* value);
+
 /* Host: per-view matrices in the reference's float arithmetic.
  * Replaces PSART::RotateVolume (Project/WaP_Tomo.cpp:1764-1797) + Scene::_calculateCamera/
  * _calculateMatrix (Project/Scene.cpp:65-189) + TypicalRayProducer::_calculateMatrix

@@ -1,0 +1,248 @@
+// Checks of the drop-in C++ classes (warp-and-project-tomography_amd/csrc/host).
+//   test_classes math IN OUT   Matrix4x4 product/inverse and Quaternion::ToMatrix on the
+//                      16+16+5 floats of IN (A, B, angles); writes A*B, inv(A) and the
+//                      five rotation matrices about +y to OUT for a bitwise comparison
+//   test_classes cpu   host-only parts: CoherentVolume bricking, Scene camera chain
+//   test_classes gpu   PSART/TypicalRayProducer/VolumeData on the device: class-built
+//                      views equal wap_view_geometry bit for bit, SART_Proximal_Operator
+//                      and WarpAndProjct equal the C ABI drivers bit for bit, Render()
+//                      runs both temporal levels
+// Prints one "ok <name>" line per passed check and exits non-zero on the first failure.
+#include <hip/hip_runtime_api.h>
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <random>
+#include <vector>
+
+#include "../../warp-and-project-tomography_amd/csrc/host/wap_classes.h"
+
+static int fails = 0;
+#define HC(call)                                                          \
+  do {                                                                    \
+    const hipError_t e_ = (call);                                         \
+    if (e_ != hipSuccess) {                                               \
+      printf("FAIL hip %s: %s\n", #call, hipGetErrorString(e_));          \
+      fflush(stdout);                                                     \
+      exit(1);                                                            \
+    }                                                                     \
+  } while (0)
+#define CHECK(cond, name)                              \
+  do {                                                 \
+    if (cond) {                                        \
+      printf("ok %s\n", name);                         \
+    } else {                                           \
+      printf("FAIL %s (line %d)\n", name, __LINE__);   \
+      ++fails;                                         \
+    }                                                  \
+    fflush(stdout);                                    \
+  } while (0)
+
+static int cpu_checks() {
+  // bricked volume: every value reproduced, uniform bricks elided, range exact
+  const int W = 37, H = 29, D = 21;
+  std::vector<float> vol((size_t)W * H * D);
+  std::mt19937 rng(7);
+  std::uniform_real_distribution<float> U(-1.0f, 2.0f);
+  for (int z = 0; z < D; ++z)
+    for (int y = 0; y < H; ++y)
+      for (int x = 0; x < W; ++x)
+        vol[x + (size_t)W * (y + (size_t)H * z)] = (x < 16 && y < 16 && z < 16) ? 0.5f : U(rng);
+  CoherentVolume cv(W, H, D);
+  const int bs[3] = {16, 16, 16};
+  cv.SetBlockSize(bs);
+  const int dims[3] = {W, H, D};
+  cv.SetDimensions(dims);
+  cv.SetData(vol.data());
+  bool same = true;
+  for (int z = 0; z < D; ++z)
+    for (int y = 0; y < H; ++y)
+      for (int x = 0; x < W; ++x)
+        same = same && cv.GetValue(x, y, z) == vol[x + (size_t)W * (y + (size_t)H * z)];
+  CHECK(same, "coherent_volume_values");
+  int bn[3];
+  cv.GetBlockNum(bn);
+  CHECK(bn[0] == 3 && bn[1] == 2 && bn[2] == 2, "coherent_volume_block_count");
+  CHECK(cv.StoredBlocks() == bn[0] * bn[1] * bn[2] - 1, "coherent_volume_uniform_elision");
+  double lo, hi;
+  cv.GetMinMaxValue(lo, hi);
+  float mn = vol[0], mx = vol[0];
+  for (float v : vol) mn = std::min(mn, v), mx = std::max(mx, v);
+  CHECK(lo == mn && hi == mx, "coherent_volume_range");
+  int gb[3];
+  cv.GetBlcokSize(gb);
+  CHECK(gb[0] == 16 && gb[1] == 16 && gb[2] == 16, "coherent_volume_block_size");
+  // Scene camera chain reproduces wap_view_geometry's view matrix path
+  Scene s;
+  s.m_sid = 96.0f;
+  s.m_sdd = 144.0f;
+  s.m_halfFrustumWidth = (40 - 1) * 1.0 * 0.5f;
+  s.m_halfFrustumHeight = (36 - 1) * 1.0 * 0.5f;
+  s.m_Rotation = Quaternion(0.0f, 1.0f, 0.0f, 0.7f) * Quaternion();
+  s._calculateCamera();
+  s._calculateMatrix();
+  CHECK(s.m_ProjectionMatrix.ele[11] == -1.0f && s.m_ViewMatrix.ele[14] == -96.0f,
+        "scene_camera");
+  return fails;
+}
+
+static int gpu_checks() {
+  const int L = 24, NU = 40, NV = 36, NV_VIEWS = 12, NF = 3;
+  int imgwh[2] = {NU, NV};
+  int vols[3] = {L, L, L};
+  double oxyz[3] = {0, 0, 0};
+  PSART ps(0.01, 0.05, 0.0, "out", imgwh, NV_VIEWS, 1, 1, 0.2, 4.0 * L, nullptr, oxyz, 6.0 * L,
+           1.0, "ATV", "Voxelbased", vols, NF, nullptr, 1);
+  std::vector<double> ang(NV_VIEWS);
+  for (int i = 0; i < NV_VIEWS; ++i) ang[i] = fmod(i * 0.6180339887498949, 1.0) * 360.0;
+  ps.SetAngles(ang.data(), NV_VIEWS);
+  VolumeData vd;
+  vd.setParas(vols, 1.0, NF, NV_VIEWS);
+  for (int q = 0; q < 8; ++q) vd.m_roi[q] = 2;
+  vd.ReadStructuredFile();
+  ps.SetData(&vd);
+  // class views == wap_view_geometry views, bit for bit
+  WapScanGeometry g;
+  memset(&g, 0, sizeof(g));
+  g.vol_dims[0] = g.vol_dims[1] = g.vol_dims[2] = L;
+  g.vol_spacing = 1.0f;
+  g.det_u = NU;
+  g.det_v = NV;
+  g.det_pitch = 1.0;
+  g.sid = 4.0f * L;
+  g.sdd = 6.0f * L;
+  g.counterclockwise = 1;
+  bool same = true;
+  for (int i = 0; i < NV_VIEWS; ++i) {
+    WapView v;
+    wap_view_geometry(&g, ang[i], &v);
+    same = same && memcmp(&v, ps.Views() + i, sizeof(WapView)) == 0;
+  }
+  CHECK(same, "psart_views_equal_c_abi_geometry");
+  // synthetic projections of a blob through the C ABI projector
+  const size_t n = (size_t)L * L * L, nd = (size_t)NU * NV;
+  std::vector<float> blob(n);
+  for (int z = 0; z < L; ++z)
+    for (int y = 0; y < L; ++y)
+      for (int x = 0; x < L; ++x) {
+        const float r2 = (x - 11.5f) * (x - 11.5f) + (y - 12.0f) * (y - 12.0f) +
+                         (z - 10.5f) * (z - 10.5f);
+        blob[x + (size_t)L * (y + (size_t)L * z)] = expf(-r2 / 18.0f);
+      }
+  float *dvol, *dsino;
+  HC(hipMalloc(&dvol, n * sizeof(float)));
+  HC(hipMalloc(&dsino, NV_VIEWS * nd * sizeof(float)));
+  HC(hipMemcpy(dvol, blob.data(), n * sizeof(float), hipMemcpyHostToDevice));
+  const int d3[3] = {L, L, L};
+  const float sd = ps.GetSampleDistance();
+  wap_forward_project(ps.Context(), dvol, 0, d3, sd, 0, NV_VIEWS, NU, NV, dsino, nullptr,
+                      nullptr, nullptr);
+  std::vector<float> sino(NV_VIEWS * nd);
+  HC(hipMemcpy(sino.data(), dsino, sino.size() * sizeof(float), hipMemcpyDeviceToHost));
+  ps.SetProjections(sino.data(), NV_VIEWS);
+  CHECK(ps.G_Max > 0.0f, "psart_gmax_from_first_projection");
+  // SART proximal step through the class == through the C ABI on a copy
+  Frame_item& f0 = vd.m_Frameslist[0][1];
+  ps.SART_Proximal_Operator(1, 2, 0, 1, 0);
+  std::vector<float> a(n), b(n);
+  f0.fVolume.download(a.data());
+  float* tmp;
+  HC(hipMalloc(&tmp, n * sizeof(float)));
+  HC(hipMemset(tmp, 0, n * sizeof(float)));
+  WapLoopConfig c;
+  memset(&c, 0, sizeof(c));
+  c.dims[0] = c.dims[1] = c.dims[2] = L;
+  c.nstates = NF + 2;
+  c.nviews = NV_VIEWS;
+  c.nu = NU;
+  c.nv = NV;
+  c.sd = sd;
+  c.alpha = 0.2;
+  c.gmax = ps.G_Max;
+  c.nb_sart = 2;
+  for (int q = 0; q < 8; ++q) c.roi[q] = c.cut[q] = 2;
+  std::vector<float> mask(n);
+  wap_roi_mask(d3, c.roi, mask.data());
+  float* dmask;
+  HC(hipMalloc(&dmask, n * sizeof(float)));
+  HC(hipMemcpy(dmask, mask.data(), n * sizeof(float), hipMemcpyHostToDevice));
+  wap_sart_prox(ps.Context(), &c, tmp, dsino, f0.fStartproj, f0.fEndproj, dmask, nullptr);
+  HC(hipMemcpy(b.data(), tmp, n * sizeof(float), hipMemcpyDeviceToHost));
+  CHECK(memcmp(a.data(), b.data(), n * sizeof(float)) == 0, "psart_sart_prox_equals_c_abi");
+  float amax = 0.0f;
+  for (float v : a) amax = std::max(amax, v);
+  CHECK(amax > 0.05f, "psart_sart_prox_reconstructs");
+  // WarpAndProjct through the class == C ABI on copies of the level storage
+  std::vector<float> lv(vd.m_LevelVolumes[0].size());
+  vd.m_LevelVolumes[0].download(lv.data());
+  float *cv, *cf;
+  HC(hipMalloc(&cv, lv.size() * sizeof(float)));
+  HC(hipMemcpy(cv, lv.data(), lv.size() * sizeof(float), hipMemcpyHostToDevice));
+  HC(hipMalloc(&cf, vd.m_LevelFlows[0].size() * sizeof(float)));
+  HC(hipMemcpy(cf, vd.m_LevelFlows[0].data(), vd.m_LevelFlows[0].size() * sizeof(float),
+               hipMemcpyDeviceToDevice));
+  int dims[3] = {L, L, L};
+  float aabb[6] = {0, (float)L, 0, (float)L, 0, (float)L};
+  ps.WarpAndProjct(5, 1, dims, 0.25f, aabb, 0, 0, nullptr, 0);
+  wap_warp_and_project(ps.Context(), &c, cv, cf, dsino, 1, 5, 0.25f, dmask, nullptr);
+  std::vector<float> lv2(lv.size());
+  vd.m_LevelVolumes[0].download(lv.data());
+  HC(hipMemcpy(lv2.data(), cv, lv2.size() * sizeof(float), hipMemcpyDeviceToHost));
+  CHECK(memcmp(lv.data(), lv2.data(), lv.size() * sizeof(float)) == 0,
+        "psart_warp_and_project_equals_c_abi");
+  // the headless outer loop over both temporal levels
+  ps.nb_volOpt[0] = 1;
+  ps.nb_volOpt[1] = 1;
+  ps.flow_iters = 10;
+  ps.flow_scales = 2;
+  ps.flow_warps = 1;
+  ps.Render();
+  CHECK(ps.LastError() == 0, "psart_render_runs");
+  std::vector<float> l1(vd.m_LevelVolumes[1].size());
+  vd.m_LevelVolumes[1].download(l1.data());
+  bool finite = true;
+  float l1max = 0.0f;
+  for (float v : l1) finite = finite && std::isfinite(v), l1max = std::max(l1max, v);
+  CHECK(finite && l1max > 0.05f, "psart_render_level1_finite_nonzero");
+  HC(hipFree(dvol));
+  HC(hipFree(dsino));
+  HC(hipFree(tmp));
+  HC(hipFree(dmask));
+  HC(hipFree(cv));
+  HC(hipFree(cf));
+  return fails;
+}
+
+static int math_io(const char* in, const char* out) {
+  float buf[37];
+  FILE* f = fopen(in, "rb");
+  if (!f || fread(buf, sizeof(float), 37, f) != 37) return 2;
+  fclose(f);
+  Matrix4x4 A, B;
+  memcpy(A.ele, buf, sizeof(A.ele));
+  memcpy(B.ele, buf + 16, sizeof(B.ele));
+  std::vector<float> o;
+  const Matrix4x4 AB = A * B;
+  o.insert(o.end(), AB.ele, AB.ele + 16);
+  Matrix4x4 inv = A;
+  inv.Inverse();
+  o.insert(o.end(), inv.ele, inv.ele + 16);
+  for (int k = 0; k < 5; ++k) {
+    const Matrix4x4 q = (Quaternion(0.0f, 1.0f, 0.0f, buf[32 + k]) * Quaternion()).ToMatrix();
+    o.insert(o.end(), q.ele, q.ele + 16);
+  }
+  f = fopen(out, "wb");
+  if (!f || fwrite(o.data(), sizeof(float), o.size(), f) != o.size()) return 2;
+  fclose(f);
+  return 0;
+}
+
+int main(int argc, char** argv) {
+  if (argc == 4 && strcmp(argv[1], "math") == 0) return math_io(argv[2], argv[3]);
+  const bool gpu = argc > 1 && strcmp(argv[1], "gpu") == 0;
+  const int r = gpu ? gpu_checks() : cpu_checks();
+  printf("%s: %d failure(s)\n", gpu ? "gpu" : "cpu", r);
+  return r == 0 ? 0 : 1;
+}

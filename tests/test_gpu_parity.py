@@ -247,3 +247,58 @@ def test_wap_iteration(wctx, oracle, W, parallel):
     np.testing.assert_array_equal(H(flows), rf)
     np.testing.assert_array_equal(H(vols), rv)
     assert rv.max() > 0.1
+
+
+@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4, 5])
+def test_flow_iteration_variants_identical(wctx, oracle, variant):
+    """Every primal-dual iteration form (two kernels, fused tiles) gives the oracle's
+    floats, on a wide grid (64-column tiles) and a narrow one (16x16 tiles)."""
+    prev = wctx.get_tuning("flow_variant")
+    wctx.set_tuning("flow_variant", variant)
+    try:
+        for shape, roi in [((20, 18, 70), [3, 2, 1, 2, 2, 3]), ((14, 16, 18), [2] * 6)]:
+            I1 = oracle.blur(helpers.blob_volume(shape, 12), 1.0)
+            I2 = oracle.advect(np.full((3,) + shape, 0.4, np.float32), I1, 1.0)
+            z3 = np.zeros((3,) + shape, np.float32)
+            z1 = np.zeros(shape, np.float32)
+            aabb = [0, shape[2], 0, shape[1], 0, shape[0]]
+            V = torch.zeros((3,) + shape, device=DEV)
+            Y = torch.zeros((10,) + shape, device=DEV)
+            it = wctx.flow_solve(T(I1), T(I2), V, Y, 0.008, 0.05, 60, 2, aabb, roi, count=True)
+            v, y1, y2, y3, y4, it_ref = oracle.flow_single(I1, I2, z3, z3, z3, z3, z1, 0.008,
+                                                           0.05, 60, 2, aabb, roi)
+            assert it == it_ref
+            np.testing.assert_array_equal(H(V), v)
+            np.testing.assert_array_equal(H(Y), np.concatenate([y1, y2, y3, y4[None]]))
+    finally:
+        wctx.set_tuning("flow_variant", prev)
+
+
+@pytest.mark.parametrize("frame", [0, 1])
+@pytest.mark.parametrize("shape,roi", [((7, 8, 9), [1] * 6), ((20, 23, 31), [2, 3, 1, 4, 2, 1])])
+def test_tvtp_step_and_xbar(wctx, oracle, frame, shape, roi):
+    """TV/temporal-prior primal-dual step and X_bar update, bit for bit."""
+    import pyoracle
+    rng = np.random.default_rng(60 + frame)
+    vol = rng.random(shape).astype(np.float32)
+    xb = rng.random(shape).astype(np.float32)
+    xn = rng.random(shape).astype(np.float32)
+    ytv = (rng.standard_normal((3,) + shape) * 0.1).astype(np.float32)
+    ytp = (rng.standard_normal(shape) * 0.1).astype(np.float32)
+    ytp_prev = (rng.standard_normal(shape) * 0.1).astype(np.float32)
+    sig, tau, tvw, tpw, gmax = 0.1125, 1.0, 0.05, 0.3, 0.9
+    o = [a.copy() for a in (vol, np.zeros(shape, np.float32), xb, xn, ytv, ytp, ytp_prev)]
+    d = pyoracle._dims(shape)
+    oracle.lib.orc_tvtp_step_f32(frame, d, np.array(roi, np.int32), o[0], o[1], o[2], o[3],
+                                 o[4], o[5], o[6], sig, tau, tvw, tpw, gmax)
+    g = [T(a) for a in (vol, np.zeros(shape, np.float32), xb, xn, ytv, ytp, ytp_prev)]
+    wctx.tvtp_step(frame, roi, g[0], g[1], g[2], g[3], g[4], g[5],
+                   g[6] if frame > 0 else None, sig, tau, tvw, tpw, gmax)
+    for k in (0, 1, 4, 5):
+        np.testing.assert_array_equal(H(g[k]), o[k])
+    # X_bar = 2x - x_prev over the ROI, untouched outside it
+    xbar = T(xb)
+    wctx.xbar_update(roi, xbar, g[0], g[1])
+    ox = xb.copy()
+    oracle.lib.orc_xbar_update_f32(d, np.array(roi, np.int32), o[0], o[1], ox)
+    np.testing.assert_array_equal(H(xbar), ox)

@@ -9,12 +9,14 @@
 //                  y4    = clip(y4 + sigma2*(2*u.v - u.v_prev + ut), -1, 1)
 // with a residual every 50 iterations deciding whether to go on.
 //
-// Storage per voxel: v (3 planes, double-buffered), y (10 planes, updated in place by
-// the dual), u (3) and ut (1). The reference keeps Kx/KxOld (22 more planes); here the
-// previous-iteration K*x terms are recomputed from the previous v, which gives the
-// same floats because the operations are identical. tau and sigma2 are recomputed
-// from u. The run/stop decision lives on the device (ctl[0]) so a whole warp of
-// iterations can be queued (or captured in a graph) without host round trips.
+// One kernel per iteration (k_flow_pd) reads v (3 planes), y (10), u (3), ut (1) and
+// writes v and y (both double-buffered): 30 floats = 120 B per voxel. The reference
+// keeps Kx/KxOld/Kty arrays besides (about 70 planes); here the previous iteration's
+// K*x terms are recomputed from the previous v and the neighbours' primal step from the
+// old y, which gives the same floats because the operations are identical. tau and
+// sigma2 are recomputed from u. The run/stop decision lives on the device (ctl[0]) so a
+// whole warp of iterations can be queued (or captured in a graph) without host round
+// trips.
 #include "wap_device.h"
 
 namespace {
@@ -83,6 +85,148 @@ __global__ void k_flow_kx4(size_t n, const float* __restrict__ u, const float* _
   kx4[t] = u[t] * v[t] + u[n + t] * v[n + t] + u[2 * n + t] * v[2 * n + t];
 }
 
+WAP_DEV bool in_roi(const FlowGrid& G, int i, int j, int k) {
+  return i >= G.r0 && i < G.W - G.r1 && j >= G.r2 && j < G.H - G.r3 && k >= G.r4 &&
+         k < G.D - G.r5;
+}
+
+// Primal update of flow component c at voxel (i, j, k) = linear index s:
+// inside the ROI  v - tau*K^T y  with  K^T y = -(Dx- y_c1 + Dy- y_c2 + Dz- y_c3) + u_c*y4,
+// outside it the unchanged v. K receives K^T y (inside the ROI).
+WAP_DEV float primal_at(const FlowGrid& G, const float* __restrict__ vold,
+                        const float* __restrict__ y, const float* __restrict__ u, size_t n, int c,
+                        int i, int j, int k, size_t s, float* K, float* tau) {
+  const float vo = vold[c * n + s];
+  if (!in_roi(G, i, j, k)) return vo;
+  const size_t sy = (size_t)G.W, sz = (size_t)G.W * G.H;
+  const float uc = u[c * n + s];
+  const float kt = -1.0f * dminus(y + (3 * c + 0) * n, s, i, G.W, 1) -
+                   1.0f * dminus(y + (3 * c + 1) * n, s, j, G.H, sy) -
+                   1.0f * dminus(y + (3 * c + 2) * n, s, k, G.D, sz) + uc * y[9 * n + s];
+  const float ta = 1.0f / (4.0f / 1.0f + my_abs(uc));
+  if (K) *K = kt;
+  if (tau) *tau = ta;
+  return vo - ta * kt;
+}
+
+// One fused primal-dual iteration. A workgroup owns a TX x TY column tile and marches
+// it through ZC planes. For every plane it computes the new v (the primal step) once
+// for the tile plus a one-voxel +x/+y halo into LDS, keeping the previous plane's, so
+// the dual step at (x, y, z) finds the new v at itself and its +x/+y/+z neighbours in
+// LDS. v and y are each read once and written once per iteration from HBM (both
+// double-buffered); u and ut are read once.
+// mode 0: previous K*x = 0 (first iteration of the solve); mode 1: previous u.v from
+// kx4prev (first iteration of a later warp); mode 2: normal.
+// resid: per-workgroup FP64 sums of the primal (ROI) and dual residual terms;
+// store_kty: keep K^T y (ROI) for the next iteration's primal residual.
+template <int TX, int TY, int ZC>
+__global__ __launch_bounds__(TX * TY) void k_flow_pd(
+    FlowGrid G, const float* __restrict__ vold, float* __restrict__ vnew,
+    const float* __restrict__ yold, float* __restrict__ ynew, const float* __restrict__ u,
+    const float* __restrict__ ut, const float* __restrict__ kx4prev, float* __restrict__ kty,
+    int mode, int resid, int store_kty, float sigma1, float lambda, double* __restrict__ ppart,
+    double* __restrict__ dpart, int* __restrict__ ctl, int iter) {
+  constexpr int PX = TX + 1, PY = TY + 1, NP = PX * PY;
+  __shared__ float pl[2][3][NP];
+  if (ctl[0] == 0) return;
+  const size_t n = (size_t)G.W * G.H * G.D;
+  const int tilesX = (G.W + TX - 1) / TX, tilesY = (G.H + TY - 1) / TY;
+  const int bid = (int)blockIdx.x;
+  const int x0 = (bid % tilesX) * TX;
+  const int y0 = ((bid / tilesX) % tilesY) * TY;
+  const int z0 = (bid / (tilesX * tilesY)) * ZC;
+  const int z1 = min(z0 + ZC, G.D);
+  const int tx = (int)threadIdx.x % TX, ty = (int)threadIdx.x / TX;
+  const int i = x0 + tx, j = y0 + ty;
+  const bool own = i < G.W && j < G.H;
+  if (bid == 0 && threadIdx.x == 0) ctl[1] = iter;
+  const float shrink_den = lambda + 2.0f * sigma1;
+  double pterm = 0.0, dterm = 0.0;
+  // primal of the plane k (tile + halo) into slot b
+  auto fill = [&](int b, int k) {
+    for (int q = (int)threadIdx.x; q < NP; q += TX * TY) {
+      const int qi = x0 + q % PX, qj = y0 + q / PX;
+      if (qi < G.W && qj < G.H) {
+        const size_t s = vidx(qi, qj, k, G.W, G.H);
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+          pl[b][c][q] = primal_at(G, vold, yold, u, n, c, qi, qj, k, s, nullptr, nullptr);
+      }
+    }
+  };
+  int cur = 0;
+  fill(cur, z0);
+  for (int k = z0; k < z1; ++k) {
+    if (k + 1 < G.D) fill(cur ^ 1, k + 1);
+    __syncthreads();
+    if (own) {
+      const size_t t = vidx(i, j, k, G.W, G.H);
+      const size_t off[3] = {1, (size_t)G.W, (size_t)G.W * G.H};
+      const bool has[3] = {i < G.W - 1, j < G.H - 1, k < G.D - 1};
+      const int q0 = tx + PX * ty;
+      const float ux = u[t], uy = u[n + t], uz = u[2 * n + t];
+      const bool roi_t = in_roi(G, i, j, k);
+      float vn[3], vo[3];
+      float pt = 0.0f, dt = 0.0f;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        vo[c] = vold[c * n + t];
+        vn[c] = pl[cur][c][q0];
+        vnew[c * n + t] = vn[c];
+        if (roi_t && (resid || store_kty)) {
+          float K0 = 0.0f, tau0 = 1.0f;
+          primal_at(G, vold, yold, u, n, c, i, j, k, t, &K0, &tau0);
+          if (resid) pt += fabsf((vo[c] - vn[c]) / tau0 - kty[c * n + t] + K0);
+          if (store_kty) kty[c * n + t] = K0;
+        }
+        const float vnb[3] = {pl[cur][c][q0 + 1], pl[cur][c][q0 + PX], pl[cur ^ 1][c][q0]};
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+          float kx = 0.0f, ko = 0.0f;
+          if (has[a]) {
+            kx = 1.0f * (vnb[a] - vn[c]);
+            if (mode != 0) ko = 1.0f * (vold[c * n + t + off[a]] - vo[c]);
+          }
+          const size_t p = (size_t)(3 * c + a) * n + t;
+          const float yo = yold[p];
+          const float yt = yo + sigma1 * (kx + kx - ko);
+          const float yn = yt * lambda / shrink_den;
+          ynew[p] = yn;
+          if (resid) dt += fabsf((yo - yn) / sigma1 - ko + kx);
+        }
+      }
+      const float k4 = ux * vn[0] + uy * vn[1] + uz * vn[2];
+      const float k4o =
+          mode == 0 ? 0.0f : (mode == 1 ? kx4prev[t] : ux * vo[0] + uy * vo[1] + uz * vo[2]);
+      const float s2 = 1.0f / (fabsf(ux) + fabsf(uy) + fabsf(uz));
+      const float y4o = yold[9 * n + t];
+      const float y4t = y4o + s2 * (k4 + k4 - k4o + ut[t]);
+      const float m1 = (y4t < 1.0f) ? y4t : 1.0f;
+      const float y4n = (-1.0f < m1) ? m1 : -1.0f;
+      ynew[9 * n + t] = y4n;
+      if (resid) {
+        dt += fabsf((y4o - y4n) / s2 - k4o + k4);
+        pterm += (double)pt;
+        dterm += (double)dt;
+      }
+    }
+    __syncthreads();
+    cur ^= 1;
+  }
+  if (resid) {
+    const double ps = block_sum<TX * TY>(pterm);
+    __syncthreads();
+    const double ds = block_sum<TX * TY>(dterm);
+    if (threadIdx.x == 0) {
+      ppart[blockIdx.x] = ps;
+      dpart[blockIdx.x] = ds;
+    }
+  }
+}
+
+// Two-kernel form of one iteration (the primal over the ROI, then the dual over every
+// voxel): more HBM traffic than the fused form but no recomputation; kept as a tuning
+// variant.
 // primal step over the ROI: vdst = vsrc - tau*K^T y  (vdst holds vsrc outside the ROI)
 __global__ __launch_bounds__(256) void k_flow_primal(
     FlowGrid G, const float* __restrict__ vsrc, float* __restrict__ vdst,
@@ -126,9 +270,9 @@ __global__ __launch_bounds__(256) void k_flow_primal(
 // mode 1: previous u.v from kx4prev (first iteration of a later warp), mode 2: normal
 __global__ __launch_bounds__(256) void k_flow_dual(
     FlowGrid G, const float* __restrict__ vold, const float* __restrict__ vnew,
-    float* __restrict__ y, const float* __restrict__ u, const float* __restrict__ ut,
+    const float* __restrict__ y, const float* __restrict__ u, const float* __restrict__ ut,
     const float* __restrict__ kx4prev, int mode, int resid, float sigma1, float lambda,
-    double* __restrict__ dpart, int* __restrict__ ctl, int iter) {
+    double* __restrict__ dpart, int* __restrict__ ctl, int iter, float* __restrict__ yout) {
   if (ctl[0] == 0) return;
   const size_t n = (size_t)G.W * G.H * G.D;
   const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -158,7 +302,7 @@ __global__ __launch_bounds__(256) void k_flow_dual(
         const float yo = y[p];
         const float yt = yo + sigma1 * (kx + kx - ko);
         const float yn = yt * lambda / shrink_den;
-        y[p] = yn;
+        yout[p] = yn;
         if (resid) term += fabsf((yo - yn) / sigma1 - ko + kx);
       }
     }
@@ -170,7 +314,7 @@ __global__ __launch_bounds__(256) void k_flow_dual(
     const float y4t = y4o + s2 * (k4 + k4 - k4o + ut[t]);
     const float m1 = (y4t < 1.0f) ? y4t : 1.0f;
     const float y4n = (-1.0f < m1) ? m1 : -1.0f;
-    y[9 * n + t] = y4n;
+    yout[9 * n + t] = y4n;
     if (resid) {
       term += fabsf((y4o - y4n) / s2 - k4o + k4);
       dterm = (double)term;
@@ -202,14 +346,24 @@ __global__ __launch_bounds__(256) void k_flow_resid(const double* __restrict__ p
   }
 }
 
-// After a warp: the host alternated v buffers for every queued iteration; if the
-// device stopped early with the other parity, move the live v into the buffer the host
-// treats as current. Re-arms the run flag for the next warp.
-__global__ void k_flow_settle(size_t n3, int host_last, float* __restrict__ host_cur,
-                              const float* __restrict__ other, int* __restrict__ ctl) {
+// After a warp: the host alternated the v and y buffers for every queued iteration;
+// if the device stopped early with the other parity, move the live state into the
+// buffers the host treats as current.
+__global__ void k_flow_settle(size_t n, int host_last, float* __restrict__ v_host,
+                              const float* __restrict__ v_other, float* __restrict__ y_host,
+                              const float* __restrict__ y_other, const int* __restrict__ ctl) {
   const int dev_last = ctl[1];
   const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (((dev_last ^ host_last) & 1) && t < n3) host_cur[t] = other[t];
+  if (!((dev_last ^ host_last) & 1) || t >= n) return;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) v_host[c * n + t] = v_other[c * n + t];
+#pragma unroll
+  for (int c = 0; c < 10; ++c) y_host[c * n + t] = y_other[c * n + t];
+}
+
+__global__ void k_zero_tail(double* __restrict__ p, size_t n) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) p[t] = 0.0;
 }
 
 __global__ void k_flow_arm(int* ctl) {
@@ -240,12 +394,29 @@ static size_t roi_count(const int* dims, const int* roi) {
   return (size_t)bx * by * bz;
 }
 
-size_t wapk_flow_primal_blocks(const int* dims, const int* roi) {
-  return (roi_count(dims, roi) + 255) / 256;
+// Tuning variants of one iteration: 0 = two kernels (primal, dual); 1..5 = the fused
+// z-marching kernel with tiles (TX, TY, ZC) below. Narrow grids (W < 64) use 16x16
+// tiles with the same ZC.
+struct PdShape {
+  int tx, ty, zc;
+};
+static const PdShape kShapes[6] = {{0, 0, 0}, {64, 4, 2}, {64, 4, 4}, {64, 4, 8},
+                                   {32, 8, 4}, {64, 4, 16}};
+
+static PdShape shape_of(int variant, const int* dims) {
+  PdShape p = kShapes[(variant >= 1 && variant <= 5) ? variant : 2];
+  if (dims[0] < 64) {
+    p.tx = 16;
+    p.ty = 16;
+  }
+  return p;
 }
 
-size_t wapk_flow_dual_blocks(const int* dims) {
-  return ((size_t)dims[0] * dims[1] * dims[2] + 255) / 256;
+size_t wapk_flow_blocks(int variant, const int* dims) {
+  if (variant == 0) return ((size_t)dims[0] * dims[1] * dims[2] + 255) / 256;
+  const PdShape p = shape_of(variant, dims);
+  return (size_t)((dims[0] + p.tx - 1) / p.tx) * ((dims[1] + p.ty - 1) / p.ty) *
+         ((dims[2] + p.zc - 1) / p.zc);
 }
 
 hipError_t wapk_flow_prep(const int* dims, const int* roi, const float* warped, const float* I1,
@@ -265,24 +436,48 @@ hipError_t wapk_flow_kx4(const int* dims, const float* u, const float* v, float*
   return hipGetLastError();
 }
 
-hipError_t wapk_flow_primal(const int* dims, const int* roi, const float* vsrc, float* vdst,
-                            const float* y, const float* u, float* kty, int store_kty,
-                            int resid, double* ppart, const int* ctl, hipStream_t s) {
-  const size_t nb = roi_count(dims, roi);
-  if (nb == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_flow_primal, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, s,
-                     mkgrid(dims, roi), vsrc, vdst, y, u, kty, store_kty, resid, ppart, ctl);
-  return hipGetLastError();
-}
+#define WAP_PD_LAUNCH(TX, TY, ZC)                                                        \
+  hipLaunchKernelGGL((k_flow_pd<TX, TY, ZC>), dim3(nb), dim3(256), 0, s, G, vold, vnew, yold, \
+                     ynew, u, ut, kx4prev, kty, mode, resid, store_kty, sigma1, lambda, ppart,  \
+                     dpart, ctl, iter)
 
-hipError_t wapk_flow_dual(const int* dims, const int* roi, const float* vold, const float* vnew,
-                          float* y, const float* u, const float* ut, const float* kx4prev,
-                          int mode, int resid, float sigma1, float lambda, double* dpart,
-                          int* ctl, int iter, hipStream_t s) {
-  const size_t n = (size_t)dims[0] * dims[1] * dims[2];
-  hipLaunchKernelGGL(k_flow_dual, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
-                     mkgrid(dims, roi), vold, vnew, y, u, ut, kx4prev, mode, resid, sigma1,
-                     lambda, dpart, ctl, iter);
+hipError_t wapk_flow_pd(int variant, const int* dims, const int* roi, const float* vold,
+                        float* vnew, const float* yold, float* ynew, const float* u,
+                        const float* ut, const float* kx4prev, float* kty, int mode, int resid,
+                        int store_kty, float sigma1, float lambda, double* ppart, double* dpart,
+                        int* ctl, int iter, hipStream_t s) {
+  const FlowGrid G = mkgrid(dims, roi);
+  const unsigned nb = (unsigned)wapk_flow_blocks(variant, dims);
+  if (variant == 0) {
+    const size_t nr = roi_count(dims, roi);
+    if (nr)
+      hipLaunchKernelGGL(k_flow_primal, dim3((unsigned)((nr + 255) / 256)), dim3(256), 0, s, G,
+                         vold, vnew, yold, u, kty, store_kty, resid, ppart, ctl);
+    if (resid) {
+      // the primal sums sit in the first ceil(nr/256) slots; clear the rest
+      const size_t used = (nr + 255) / 256;
+      if (nb > used)
+        hipLaunchKernelGGL(k_zero_tail, dim3((unsigned)((nb - used + 255) / 256)), dim3(256), 0,
+                           s, ppart + used, nb - used);
+    }
+    hipLaunchKernelGGL(k_flow_dual, dim3(nb), dim3(256), 0, s, G, vold, vnew, yold, u, ut,
+                       kx4prev, mode, resid, sigma1, lambda, dpart, ctl, iter, ynew);
+    return hipGetLastError();
+  }
+  const PdShape p = shape_of(variant, dims);
+  if (p.tx == 16) {
+    if (p.zc == 2) WAP_PD_LAUNCH(16, 16, 2);
+    else if (p.zc == 4) WAP_PD_LAUNCH(16, 16, 4);
+    else if (p.zc == 8) WAP_PD_LAUNCH(16, 16, 8);
+    else WAP_PD_LAUNCH(16, 16, 16);
+  } else if (p.tx == 32) {
+    WAP_PD_LAUNCH(32, 8, 4);
+  } else {
+    if (p.zc == 2) WAP_PD_LAUNCH(64, 4, 2);
+    else if (p.zc == 4) WAP_PD_LAUNCH(64, 4, 4);
+    else if (p.zc == 8) WAP_PD_LAUNCH(64, 4, 8);
+    else WAP_PD_LAUNCH(64, 4, 16);
+  }
   return hipGetLastError();
 }
 
@@ -293,10 +488,10 @@ hipError_t wapk_flow_resid(const double* ppart, int np, const double* dpart, int
   return hipGetLastError();
 }
 
-hipError_t wapk_flow_settle(size_t n3, int host_last, float* host_cur, const float* other,
-                            int* ctl, hipStream_t s) {
-  hipLaunchKernelGGL(k_flow_settle, dim3((unsigned)((n3 + 255) / 256)), dim3(256), 0, s, n3,
-                     host_last, host_cur, other, ctl);
+hipError_t wapk_flow_settle(size_t n, int host_last, float* v_host, const float* v_other,
+                            float* y_host, const float* y_other, const int* ctl, hipStream_t s) {
+  hipLaunchKernelGGL(k_flow_settle, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n,
+                     host_last, v_host, v_other, y_host, y_other, ctl);
   return hipGetLastError();
 }
 

@@ -1,0 +1,844 @@
+// Drop-in C++ classes over the wap C ABI (see wap_classes.h for the reference map).
+#include "wap_classes.h"
+
+#include <hip/hip_runtime_api.h>
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <fstream>
+#include <stdexcept>
+
+// ---------------------------------------------------------------------------------
+// math (Project/Matrix4x4.cpp:80-135, Project/Quaternion.h:46-70)
+// ---------------------------------------------------------------------------------
+static inline float cof3(float a, float b, float c, float d, float e, float f, float g, float h,
+                         float i) {
+  return a * e * i + b * f * g + c * d * h - c * e * g - a * f * h - b * d * i;
+}
+
+float Matrix4x4::Inverse() {
+  const float* s = ele;
+  const float a1 = s[0], b1 = s[1], c1 = s[2], d1 = s[3], a2 = s[4], b2 = s[5], c2 = s[6],
+              d2 = s[7], a3 = s[8], b3 = s[9], c3 = s[10], d3 = s[11], a4 = s[12], b4 = s[13],
+              c4 = s[14], d4 = s[15];
+  const float k1 = cof3(b2, b3, b4, c2, c3, c4, d2, d3, d4);
+  const float k2 = cof3(a2, a3, a4, c2, c3, c4, d2, d3, d4);
+  const float k3 = cof3(a2, a3, a4, b2, b3, b4, d2, d3, d4);
+  const float k4 = cof3(a2, a3, a4, b2, b3, b4, c2, c3, c4);
+  const float det = a1 * k1 - b1 * k2 + c1 * k3 - d1 * k4;
+  if (det == 0.0f) return 0.0f;
+  const float q = 1.0f / det;
+  float* o = ele;
+  o[0] = k1 * q;
+  o[4] = -k2 * q;
+  o[8] = k3 * q;
+  o[12] = -k4 * q;
+  o[1] = -cof3(b1, b3, b4, c1, c3, c4, d1, d3, d4) * q;
+  o[5] = cof3(a1, a3, a4, c1, c3, c4, d1, d3, d4) * q;
+  o[9] = -cof3(a1, a3, a4, b1, b3, b4, d1, d3, d4) * q;
+  o[13] = cof3(a1, a3, a4, b1, b3, b4, c1, c3, c4) * q;
+  o[2] = cof3(b1, b2, b4, c1, c2, c4, d1, d2, d4) * q;
+  o[6] = -cof3(a1, a2, a4, c1, c2, c4, d1, d2, d4) * q;
+  o[10] = cof3(a1, a2, a4, b1, b2, b4, d1, d2, d4) * q;
+  o[14] = -cof3(a1, a2, a4, b1, b2, b4, c1, c2, c4) * q;
+  o[3] = -cof3(b1, b2, b3, c1, c2, c3, d1, d2, d3) * q;
+  o[7] = cof3(a1, a2, a3, c1, c2, c3, d1, d2, d3) * q;
+  o[11] = -cof3(a1, a2, a3, b1, b2, b3, d1, d2, d3) * q;
+  o[15] = cof3(a1, a2, a3, b1, b2, b3, c1, c2, c3) * q;
+  return det;
+}
+
+Matrix4x4 Quaternion::ToMatrix() const {
+  Matrix4x4 mat;
+  float* m = mat.ele;
+  m[0] = 1.0f - 2.0f * (y * y + z * z);
+  m[1] = 2.0f * (x * y + z * w);
+  m[2] = 2.0f * (z * x - y * w);
+  m[3] = 0.0f;
+  m[4] = 2.0f * (x * y - z * w);
+  m[5] = 1.0f - 2.0f * (z * z + x * x);
+  m[6] = 2.0f * (y * z + x * w);
+  m[7] = 0.0f;
+  m[8] = 2.0f * (z * x + y * w);
+  m[9] = 2.0f * (y * z - x * w);
+  m[10] = 1.0f - 2.0f * (x * x + y * y);
+  m[11] = 0.0f;
+  m[12] = m[13] = m[14] = 0.0f;
+  m[15] = 1.0f;
+  return mat;
+}
+
+// ---------------------------------------------------------------------------------
+// DeviceImage
+// ---------------------------------------------------------------------------------
+static void hip_check(hipError_t e, const char* what) {
+  if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+static void fill_device(float* p, size_t n, float v) {
+  if (n == 0) return;
+  if (v == 0.0f) {
+    hip_check(hipMemset(p, 0, n * sizeof(float)), "hipMemset");
+    return;
+  }
+  std::vector<float> h(n, v);
+  hip_check(hipMemcpy(p, h.data(), n * sizeof(float), hipMemcpyHostToDevice), "hipMemcpy");
+}
+
+DeviceImage::DeviceImage(int w, int h, int d, int c, float fill) { assign(w, h, d, c, fill); }
+
+DeviceImage::DeviceImage(const DeviceImage& o) { *this = o; }
+
+DeviceImage DeviceImage::View(float* data, int w, int h, int d, int c) {
+  DeviceImage v;
+  v.data_ = data;
+  v.w_ = w;
+  v.h_ = h;
+  v.d_ = d;
+  v.c_ = c;
+  v.owner_ = false;
+  return v;
+}
+
+void DeviceImage::assign(int w, int h, int d, int c, float fill) {
+  if (owner_ && data_) (void)hipFree(data_);
+  data_ = nullptr;
+  owner_ = false;
+  w_ = w;
+  h_ = h;
+  d_ = d;
+  c_ = c;
+  if (size() == 0) return;
+  hip_check(hipMalloc(&data_, size() * sizeof(float)), "hipMalloc");
+  owner_ = true;
+  fill_device(data_, size(), fill);
+}
+
+DeviceImage& DeviceImage::operator=(const DeviceImage& o) {
+  if (this == &o) return *this;
+  if (o.is_empty()) {
+    if (owner_) {
+      assign(0, 0, 0, 0);
+    } else if (data_) {
+      // a view keeps its storage; an empty source leaves it unchanged (CImg no-op)
+    }
+    return *this;
+  }
+  if (!data_ || size() != o.size()) {
+    if (!owner_ && data_)
+      throw std::runtime_error("DeviceImage: size mismatch when assigning into a view");
+    assign(o.w_, o.h_, o.d_, o.c_);
+  }
+  hip_check(hipMemcpy(data_, o.data_, size() * sizeof(float), hipMemcpyDeviceToDevice),
+            "hipMemcpy");
+  return *this;
+}
+
+DeviceImage& DeviceImage::operator=(float value) {
+  if (data_) fill_device(data_, size(), value);
+  return *this;
+}
+
+DeviceImage::~DeviceImage() {
+  if (owner_ && data_) (void)hipFree(data_);
+}
+
+void DeviceImage::upload(const float* host) {
+  hip_check(hipMemcpy(data_, host, size() * sizeof(float), hipMemcpyHostToDevice), "upload");
+}
+
+void DeviceImage::download(float* host) const {
+  hip_check(hipMemcpy(host, data_, size() * sizeof(float), hipMemcpyDeviceToHost), "download");
+}
+
+// ---------------------------------------------------------------------------------
+// Frame_item / VolumeData (Project/Frames.h:16-63, Project/VolumeData.cpp:43-228)
+// ---------------------------------------------------------------------------------
+Frame_item::Frame_item(int volsize[3]) {
+  fTime = 0;
+  fVolume = image(volsize[0], volsize[1], volsize[2], 1, 0.0f);
+  fStartproj = -1;
+  fEndproj = -1;
+  SetVolSize(volsize);
+}
+
+void Frame_item::SetVolSize(int volsize[3]) {
+  fsize[0] = volsize[0];
+  fsize[1] = volsize[1];
+  fsize[2] = volsize[2];
+}
+
+void Frame_item::GetVolSize(int* volsize) {
+  volsize[0] = fsize[0];
+  volsize[1] = fsize[1];
+  volsize[2] = fsize[2];
+}
+
+VolumeData::VolumeData() = default;
+
+VolumeData::VolumeData(int d1, int d2, int d3, float sp1, float sp2, float sp3, float* data) {
+  m_Dims[0] = m_volX = d1;
+  m_Dims[1] = m_volY = d2;
+  m_Dims[2] = m_volZ = d3;
+  m_Spacings[0] = sp1;
+  m_Spacings[1] = sp2;
+  m_Spacings[2] = sp3;
+  (void)data;
+}
+
+VolumeData::~VolumeData() = default;
+
+void VolumeData::Clean() {
+  for (auto& l : m_Frameslist) l.clear();
+  for (auto& v : m_LevelVolumes) v.assign(0, 0, 0, 0);
+  for (auto& f : m_LevelFlows) f.assign(0, 0, 0, 0);
+}
+
+void VolumeData::GetDimensions(int* dims) const {
+  dims[0] = m_Dims[0];
+  dims[1] = m_Dims[1];
+  dims[2] = m_Dims[2];
+}
+
+void VolumeData::SetDimensions(int* dims) {
+  m_Dims[0] = dims[0];
+  m_Dims[1] = dims[1];
+  m_Dims[2] = dims[2];
+}
+
+void VolumeData::GetSpacings(float* s) const {
+  s[0] = m_Spacings[0];
+  s[1] = m_Spacings[1];
+  s[2] = m_Spacings[2];
+}
+
+void VolumeData::SetSpacings(const float* s) {
+  m_Spacings[0] = s[0];
+  m_Spacings[1] = s[1];
+  m_Spacings[2] = s[2];
+}
+
+void VolumeData::setParas(int* vols, double vs, int nframes, int totalproj) {
+  m_volX = vols[0];
+  m_volY = vols[1];
+  m_volZ = vols[2];
+  Vol_Spacing = vs;
+  m_nframes = nframes;
+  m_totalProjs = totalproj;
+}
+
+void VolumeData::ReadStructuredFile() {
+  m_Dims[0] = m_volX;
+  m_Dims[1] = m_volY;
+  m_Dims[2] = m_volZ;
+  m_Spacings[0] = m_Spacings[1] = m_Spacings[2] = (float)Vol_Spacing;
+  size_item sz;
+  sz.volsize[0] = m_Dims[0];
+  sz.volsize[1] = m_Dims[1];
+  sz.volsize[2] = m_Dims[2];
+  m_Volumesize.push_back(sz);
+  const int maxFrames = 130;
+  const size_t n = (size_t)m_Dims[0] * m_Dims[1] * m_Dims[2];
+  int factor = 1;
+  for (int level = 1; level <= m_levels; ++level) {
+    const int nf = std::min(m_nframes * factor, maxFrames);
+    m_fInLevel[level - 1] = nf;
+    const float timeunit = (float)((float)m_totalProjs / (float)nf);
+    m_tUnitInLevel[level - 1] = timeunit;
+    const float baseNo = 3;
+    m_base = (int)baseNo;
+    const int total = nf + 2;
+    m_LevelVolumes[level - 1].assign(m_Dims[0], m_Dims[1], m_Dims[2], total, 0.0f);
+    m_LevelFlows[level - 1].assign(m_Dims[0], m_Dims[1], m_Dims[2], 3 * total, 0.0f);
+    std::vector<Frame_item>& L = m_Frameslist[level - 1];
+    L.clear();
+    for (int i = 0; i < total; ++i) {
+      const int src = std::min(i, nf - 1);  // two duplicated tail frames
+      Frame_item it;
+      it.SetVolSize(m_Dims);
+      it.fVolume = image::View(m_LevelVolumes[level - 1].data() + (size_t)i * n, m_Dims[0],
+                               m_Dims[1], m_Dims[2], 1);
+      it.fFlow = image::View(m_LevelFlows[level - 1].data() + (size_t)i * 3 * n, m_Dims[0],
+                             m_Dims[1], m_Dims[2], 3);
+      it.fTime = timeunit * src;
+      it.fStartproj = std::max(0, (int)(it.fTime - baseNo));
+      it.fEndproj = std::min((int)(it.fTime + baseNo), m_totalProjs - 1);
+      it.fprojRange = it.fEndproj - it.fStartproj;
+      L.push_back(it);
+    }
+    factor *= 2;
+  }
+}
+
+// ---------------------------------------------------------------------------------
+// RayCaster / TypicalRayProducer (Project/TypicalRayProducer.cpp:27-262)
+// ---------------------------------------------------------------------------------
+RayCaster::RayCaster() {
+  m_LightDirection = Vector4(0.0f, 0.0f, 1.0f, 0.0f);
+  m_OldLightDirection = m_LightDirection;
+}
+
+RayCaster::~RayCaster() = default;
+
+TypicalRayProducer::TypicalRayProducer(int volX, int volY, int volZ, int imgW, int imgH,
+                                       double ds, const char* bp) {
+  (void)ds;
+  m_VolDims[0] = volX;
+  m_VolDims[1] = volY;
+  m_VolDims[2] = volZ;
+  m_ViewPort[0] = imgW;
+  m_ViewPort[1] = imgH;
+  m_RayBased = bp && strcmp(bp, "Raybased") == 0;
+  const size_t nd = (size_t)imgW * imgH;
+  const size_t nv = (size_t)volX * volY * volZ;
+  float** imgs[4] = {&m_ImageData, &m_WeightOfImage, &m_imageraylength, &m_Correction};
+  for (float** p : imgs) {
+    hip_check(hipMalloc(p, nd * sizeof(float)), "hipMalloc");
+    fill_device(*p, nd, 0.0f);
+  }
+  if (m_RayBased) {
+    hip_check(hipMalloc(&m_EachVoxel_TotalWeight, nv * sizeof(float)), "hipMalloc");
+    hip_check(hipMalloc(&m_Voxel_Contributions, nv * sizeof(float)), "hipMalloc");
+    fill_device(m_EachVoxel_TotalWeight, nv, 0.0f);
+    fill_device(m_Voxel_Contributions, nv, 0.0f);
+  }
+}
+
+TypicalRayProducer::~TypicalRayProducer() {
+  for (float* p : {m_ImageData, m_WeightOfImage, m_imageraylength, m_Correction,
+                   m_EachVoxel_TotalWeight, m_Voxel_Contributions})
+    if (p) (void)hipFree(p);
+}
+
+void TypicalRayProducer::Initial() {
+  m_BackProjection = false;
+  const size_t nd = (size_t)m_ViewPort[0] * m_ViewPort[1];
+  for (float* p : {m_imageraylength, m_ImageData, m_WeightOfImage, m_Correction})
+    fill_device(p, nd, 0.0f);
+  if (m_RayBased) {
+    const size_t nv = (size_t)m_VolDims[0] * m_VolDims[1] * m_VolDims[2];
+    fill_device(m_EachVoxel_TotalWeight, nv, 0.0f);
+    fill_device(m_Voxel_Contributions, nv, 0.0f);
+  }
+}
+
+void TypicalRayProducer::SetViewPort(int width, int height) {
+  m_ViewPort[0] = width;
+  m_ViewPort[1] = height;
+}
+
+void TypicalRayProducer::SetVolSize(const int volDims[3], const float volSpacings) {
+  for (int a = 0; a < 3; ++a) {
+    m_VolDims[a] = volDims[a];
+    m_VolSpacings[a] = volSpacings;
+    m_VolSize[a] = (volDims[a] - 1) * volSpacings;
+  }
+}
+
+void TypicalRayProducer::GetVolSize(int volDims[3], float volSpacings) {
+  (void)volSpacings;
+  volDims[0] = m_VolDims[0];
+  volDims[1] = m_VolDims[1];
+  volDims[2] = m_VolDims[2];
+}
+
+void TypicalRayProducer::_calculateMatrix() {
+  const Matrix4x4 mv =
+      m_ViewMatrix *
+      Matrix4x4::TranslateMatrix(-m_VolSize[0] * 0.5f, -m_VolSize[1] * 0.5f, -m_VolSize[2] * 0.5f);
+  m_model_view = mv;
+  Matrix4x4 inv = mv;
+  inv.Inverse();
+  m_CameraPos[0] = inv[12] / inv[15];
+  m_CameraPos[1] = inv[13] / inv[15];
+  m_CameraPos[2] = inv[14] / inv[15];
+  m_ModelToVolumeMatrix.IdentityMatrix();
+  m_ModelToVolumeMatrix.ele[0] = 1.0f / m_VolSpacings[0];
+  m_ModelToVolumeMatrix.ele[5] = 1.0f / m_VolSpacings[1];
+  m_ModelToVolumeMatrix.ele[10] = 1.0f / m_VolSpacings[2];
+  m_VolumeToModelMatrix = m_ModelToVolumeMatrix;
+  m_VolumeToModelMatrix.Inverse();
+  Matrix4x4 toPix;
+  toPix.ele[0] = 0.5f * (m_ViewPort[0] - 1);
+  toPix.ele[5] = 0.5f * (m_ViewPort[1] - 1);
+  toPix.ele[10] = 0.5f;
+  toPix.ele[12] = toPix.ele[0];
+  toPix.ele[13] = toPix.ele[5];
+  toPix.ele[14] = toPix.ele[10];
+  toPix.ele[15] = 1.0f;
+  m_PixelsToViewMatrix = toPix;
+  m_PixelsToViewMatrix.Inverse();
+  m_ModelToPixelsMatrix = toPix * m_ProjectionMatrix * mv;
+  m_PixelsToModelMatrix = m_ModelToPixelsMatrix;
+  m_PixelsToModelMatrix.Inverse();
+}
+
+void TypicalRayProducer::UpdateMatrixAndBounds() {
+  _calculateMatrix();
+  m_ImageOrigin[0] = m_ImageOrigin[1] = 0;
+  m_ImageInUseSize[0] = m_ViewPort[0];
+  m_ImageInUseSize[1] = m_ViewPort[1];
+}
+
+void TypicalRayProducer::GetImageInUseSize(int s[2]) {
+  s[0] = m_ImageInUseSize[0];
+  s[1] = m_ImageInUseSize[1];
+}
+
+void TypicalRayProducer::GetImageOrigin(int o[2]) {
+  o[0] = m_ImageOrigin[0];
+  o[1] = m_ImageOrigin[1];
+}
+
+WapView TypicalRayProducer::CurrentView(float sdd, double half_w, double half_h, double ds) const {
+  WapView v;
+  memset(&v, 0, sizeof(v));
+  memcpy(v.p2m, m_PixelsToModelMatrix.ele, sizeof(v.p2m));
+  memcpy(v.m2v, m_ModelToVolumeMatrix.ele, sizeof(v.m2v));
+  const Matrix4x4 a = m_model_view * m_VolumeToModelMatrix;
+  memcpy(v.mvv2m, a.ele, sizeof(v.mvv2m));
+  v.eye[0] = v.p2m[8] / v.p2m[11];
+  v.eye[1] = v.p2m[9] / v.p2m[11];
+  v.eye[2] = v.p2m[10] / v.p2m[11];
+  v.eye[3] = 1.0f;
+  v.half_w = half_w;
+  v.half_h = half_h;
+  v.ds = ds;
+  v.sdd = sdd;
+  v.parallel = 0;
+  return v;
+}
+
+// ---------------------------------------------------------------------------------
+// CoherentVolume (Project/CoherentVolume.cpp:19-283)
+// ---------------------------------------------------------------------------------
+class CoherentVolume::BlockArray {
+ public:
+  struct Block {
+    float minv = 0.0f, maxv = 0.0f;
+    std::vector<float> data;  // empty when the block is uniform
+  };
+  std::vector<Block> blocks;
+};
+
+CoherentVolume::CoherentVolume(int x, int y, int z) {
+  const int d[3] = {x, y, z};
+  SetDimensions(d);
+}
+
+CoherentVolume::~CoherentVolume() { delete m_BlockArray; }
+
+void CoherentVolume::Initialize() { AllocateBlocks(); }
+
+void CoherentVolume::SetDimensions(const int dims[3]) {
+  for (int a = 0; a < 3; ++a) {
+    m_Dimensions[a] = dims[a];
+    m_BlockNum[a] = (m_Dimensions[a] - 1) / m_BlockSize[a] + 1;
+  }
+}
+
+void CoherentVolume::GetDimensions(int dims[3]) const {
+  for (int a = 0; a < 3; ++a) dims[a] = m_Dimensions[a];
+}
+
+void CoherentVolume::SetSpacings(const float s[3]) {
+  for (int a = 0; a < 3; ++a) m_Spacings[a] = s[a];
+}
+
+void CoherentVolume::GetSpacings(float s[3]) const {
+  for (int a = 0; a < 3; ++a) s[a] = m_Spacings[a];
+}
+
+void CoherentVolume::GetBlockNum(int bn[3]) {
+  for (int a = 0; a < 3; ++a) {
+    m_BlockNum[a] = (m_Dimensions[a] - 1) / m_BlockSize[a] + 1;
+    bn[a] = m_BlockNum[a];
+  }
+}
+
+void CoherentVolume::AllocateBlocks() {
+  int bn[3];
+  GetBlockNum(bn);
+  delete m_BlockArray;
+  m_BlockArray = new BlockArray();
+  m_BlockArray->blocks.resize((size_t)bn[0] * bn[1] * bn[2]);
+  const size_t bs = (size_t)m_BlockSize[0] * m_BlockSize[1] * m_BlockSize[2];
+  for (auto& b : m_BlockArray->blocks) b.data.assign(bs, 0.0f);
+}
+
+// One z-slice of the volume (W*H floats, x fastest) into the bricks it crosses.
+void CoherentVolume::SetBlockSlice(void* sliceData, int sliceID, int dimz) {
+  (void)dimz;
+  if (!m_BlockArray) AllocateBlocks();
+  const float* s = (const float*)sliceData;
+  const int bz = sliceID / m_BlockSize[2], lz = sliceID % m_BlockSize[2];
+  for (int y = 0; y < m_Dimensions[1]; ++y)
+    for (int x = 0; x < m_Dimensions[0]; ++x) {
+      const int bx = x / m_BlockSize[0], by = y / m_BlockSize[1];
+      auto& b = m_BlockArray->blocks[bx + (size_t)m_BlockNum[0] * (by + (size_t)m_BlockNum[1] * bz)];
+      const int lx = x % m_BlockSize[0], ly = y % m_BlockSize[1];
+      b.data[lx + (size_t)m_BlockSize[0] * (ly + (size_t)m_BlockSize[1] * lz)] =
+          s[x + (size_t)m_Dimensions[0] * y];
+    }
+}
+
+void CoherentVolume::SetData(float* data) {
+  AllocateBlocks();
+  const size_t slice = (size_t)m_Dimensions[0] * m_Dimensions[1];
+  for (int z = 0; z < m_Dimensions[2]; ++z) SetBlockSlice(data + slice * z, z);
+  // per-block range over the voxels inside the volume; uniform blocks drop their data
+  for (int bz = 0; bz < m_BlockNum[2]; ++bz)
+    for (int by = 0; by < m_BlockNum[1]; ++by)
+      for (int bx = 0; bx < m_BlockNum[0]; ++bx) {
+        auto& b = m_BlockArray->blocks[bx + (size_t)m_BlockNum[0] * (by + (size_t)m_BlockNum[1] * bz)];
+        bool first = true;
+        for (int lz = 0; lz < m_BlockSize[2]; ++lz)
+          for (int ly = 0; ly < m_BlockSize[1]; ++ly)
+            for (int lx = 0; lx < m_BlockSize[0]; ++lx) {
+              const int x = bx * m_BlockSize[0] + lx, y = by * m_BlockSize[1] + ly,
+                        z = bz * m_BlockSize[2] + lz;
+              if (x >= m_Dimensions[0] || y >= m_Dimensions[1] || z >= m_Dimensions[2]) continue;
+              const float v =
+                  b.data[lx + (size_t)m_BlockSize[0] * (ly + (size_t)m_BlockSize[1] * lz)];
+              if (first) {
+                b.minv = b.maxv = v;
+                first = false;
+              } else {
+                b.minv = std::min(b.minv, v);
+                b.maxv = std::max(b.maxv, v);
+              }
+            }
+        if (b.minv == b.maxv) std::vector<float>().swap(b.data);
+      }
+}
+
+float CoherentVolume::GetValue(int x, int y, int z) {
+  const int bx = x / m_BlockSize[0], by = y / m_BlockSize[1], bz = z / m_BlockSize[2];
+  const auto& b =
+      m_BlockArray->blocks[bx + (size_t)m_BlockNum[0] * (by + (size_t)m_BlockNum[1] * bz)];
+  if (b.data.empty()) return b.minv;
+  const int lx = x % m_BlockSize[0], ly = y % m_BlockSize[1], lz = z % m_BlockSize[2];
+  return b.data[lx + (size_t)m_BlockSize[0] * (ly + (size_t)m_BlockSize[1] * lz)];
+}
+
+void CoherentVolume::GetMinMaxValue(double& minValue, double& maxValue) {
+  bool first = true;
+  for (const auto& b : m_BlockArray->blocks) {
+    if (first) {
+      minValue = b.minv;
+      maxValue = b.maxv;
+      first = false;
+    } else {
+      minValue = std::min(minValue, (double)b.minv);
+      maxValue = std::max(maxValue, (double)b.maxv);
+    }
+  }
+}
+
+int CoherentVolume::StoredBlocks() const {
+  int k = 0;
+  if (m_BlockArray)
+    for (const auto& b : m_BlockArray->blocks) k += b.data.empty() ? 0 : 1;
+  return k;
+}
+
+// ---------------------------------------------------------------------------------
+// Scene (Project/Scene.cpp:10-189: camera and matrices)
+// ---------------------------------------------------------------------------------
+Scene::Scene() { m_ds = 1.0f; }
+
+void Scene::Resize(int width, int height) {
+  m_ViewPort[0] = width;
+  m_ViewPort[1] = height;
+  m_NeedCalculateMatrix = true;
+}
+
+void Scene::SetSize3DModified() {
+  m_Scale = 1.0f;
+  m_Rotation.Identity();
+  m_NeedCalculateCamera = true;
+}
+
+void Scene::_calculateCamera() {
+  m_ZTranslateBase = -1 * m_sid;
+  m_Near = m_sdd;
+  m_Far = 10000.0;
+  m_NeedCalculateMatrix = true;
+}
+
+void Scene::_calculateMatrix() {
+  m_ViewMatrix = Matrix4x4::TranslateMatrix(m_Translate[0], m_Translate[1],
+                                            m_ZTranslateBase + m_Translate[2]) *
+                 Matrix4x4::ScaleMatrix(m_Scale, m_Scale, m_Scale) * m_Rotation.ToMatrix();
+  float* p = m_ProjectionMatrix.ele;
+  p[0] = (float)(m_Near / m_halfFrustumWidth);
+  p[1] = p[2] = p[3] = 0.0f;
+  p[4] = 0.0f;
+  p[5] = (float)(m_Near / m_halfFrustumHeight);
+  p[6] = p[7] = p[8] = p[9] = 0.0f;
+  p[10] = -(m_Far + m_Near) / (m_Far - m_Near);
+  p[11] = -1.0f;
+  p[12] = p[13] = 0.0f;
+  p[14] = -2.0f * m_Far * m_Near / (m_Far - m_Near);
+  p[15] = 0.0f;
+}
+
+// ---------------------------------------------------------------------------------
+// PSART (Project/WaP_Tomo.cpp:228-877, 1207-1797)
+// ---------------------------------------------------------------------------------
+PSART::PSART(double d1, double d2, double d3, const char* outputfile, int* imgwh, int projNo,
+             int iters, int sartiters, double alp, double sid, const char** projsfile,
+             double* oxyz, double sdd, double ds, const char* prior, const char* bp, int* volxyz,
+             int nframes, double* startdg, int nrounds) {
+  (void)d3;
+  (void)iters;
+  (void)sartiters;
+  (void)prior;
+  (void)startdg;
+  m_nframes = nframes;
+  m_nrounds = nrounds;
+  m_RayProducer = new TypicalRayProducer(volxyz[0], volxyz[1], volxyz[2], imgwh[0], imgwh[1], ds,
+                                         bp ? bp : "Voxelbased");
+  m_alpha = alp;
+  TV_w = (float)d1;
+  TP_w = (float)d2;
+  m_halfFrustumWidth = (imgwh[0] - 1) * ds * 0.5f;
+  m_halfFrustumHeight = (imgwh[1] - 1) * ds * 0.5f;
+  m_projNo = projNo;
+  m_sid = (float)sid;
+  m_sdd = (float)sdd;
+  m_ds = ds;
+  m_ViewPort[0] = imgwh[0];
+  m_ViewPort[1] = imgwh[1];
+  if (oxyz) {
+    m_Translate[0] = (float)-oxyz[0];
+    m_Translate[2] = (float)+oxyz[1];
+    m_Translate[1] = (float)-oxyz[2];
+  }
+  m_output = outputfile ? outputfile : "";
+  m_RayCaster = new RayCaster;
+  m_RayProducer->SetViewPort(imgwh[0], imgwh[1]);
+  m_angles.assign(projNo, 0.0);
+  {
+    // the reference reads its angle list from ours_foam_compress.txt in the working
+    // directory (Project/WaP_Tomo.cpp:324-331); use it when present
+    std::ifstream in("ours_foam_compress.txt");
+    for (int i = 0; in && i < projNo; ++i) in >> m_angles[i];
+  }
+  Ok(wap_ctx_create(&m_ctx, 0));
+  m_sino.assign(imgwh[0], imgwh[1], projNo, 1, 0.0f);
+  if (projsfile && projsfile[0]) {
+    // raw float32 stack of projNo images of W x H (the reference loads a TIFF stack)
+    std::ifstream f(projsfile[0], std::ios::binary);
+    std::vector<float> h(m_sino.size());
+    if (f.read((char*)h.data(), h.size() * sizeof(float))) SetProjections(h.data(), projNo);
+  }
+}
+
+PSART::~PSART() {
+  delete m_RayProducer;
+  delete m_RayCaster;
+  if (m_ctx) wap_ctx_destroy(m_ctx);
+}
+
+bool PSART::Ok(int r) {
+  if (r != WAP_OK) {
+    m_err = r;
+    fprintf(stderr, "wap: %s\n", wap_last_error(m_ctx));
+    return false;
+  }
+  return true;
+}
+
+int PSART::SetProjections(const float* host_stack, int nviews) {
+  if (nviews != m_projNo) return WAP_E_ARG;
+  m_sino.upload(host_stack);
+  // G_Max = max of the first projection (Project/WaP_Tomo.cpp:347-349)
+  const size_t nd = (size_t)m_ViewPort[0] * m_ViewPort[1];
+  float mx = host_stack[0];
+  for (size_t k = 1; k < nd; ++k) mx = std::max(mx, host_stack[k]);
+  G_Max = mx / 1.0f;
+  return WAP_OK;
+}
+
+int PSART::SetAngles(const double* degrees, int n) {
+  if (n != m_projNo) return WAP_E_ARG;
+  m_angles.assign(degrees, degrees + n);
+  if (m_Data) BuildViews();
+  return WAP_OK;
+}
+
+bool PSART::RotateVolume(int i) {
+  m_NeedCalculateMatrix = true;
+  m_NeedCalculateCamera = true;
+  const double angle = m_angles[i] * 3.1415926 / 180;
+  Vector4 v;
+  v[0] = 0;
+  v[1] = 1;
+  v[2] = v[3] = 0.0f;
+  const double length = v.Length();
+  v *= (float)(1.0f / length);
+  m_Rotation = Quaternion(v[0], v[1], v[2], (float)(length * angle)) * m_OldRotation;
+  return true;
+}
+
+void PSART::BuildViews() {
+  m_views.resize(m_projNo);
+  for (int i = 0; i < m_projNo; ++i) {
+    RotateVolume(i);
+    _calculateCamera();
+    _calculateMatrix();
+    m_NeedCalculateCamera = m_NeedCalculateMatrix = false;
+    m_RayProducer->SetViewMatrix(GetViewMatrix());
+    m_RayProducer->SetProjectionMatrix(GetProjectionMatrix());
+    m_RayProducer->UpdateMatrixAndBounds();
+    m_views[i] = m_RayProducer->CurrentView(m_sdd, m_halfFrustumWidth, m_halfFrustumHeight, m_ds);
+  }
+  Ok(wap_upload_views(m_ctx, m_views.data(), m_projNo));
+}
+
+void PSART::SetData(VolumeData* data) {
+  m_Data = data;
+  SetSize3DModified();
+  if (!data) return;
+  float sp[3];
+  m_Data->GetDimensions(m_voldims);
+  m_Data->GetSpacings(sp);
+  m_volspace = sp[1];
+  m_RayProducer->SetVolSize(m_voldims, sp[1]);
+  m_RayCaster->SetSampleDistance(sqrtf(sp[0] * sp[0] + sp[1] * sp[1] + sp[2] * sp[2]) / 2);
+  // imROI from the explicit m_roi (the reference builds it before m_roi is set)
+  std::vector<float> mask((size_t)m_voldims[0] * m_voldims[1] * m_voldims[2]);
+  wap_roi_mask(m_voldims, m_Data->m_roi, mask.data());
+  m_roiMask.assign(m_voldims[0], m_voldims[1], m_voldims[2], 1);
+  m_roiMask.upload(mask.data());
+  BuildViews();
+}
+
+void PSART::SetSampleDistance(float sd) { m_RayCaster->SetSampleDistance(sd); }
+
+float PSART::GetSampleDistance() { return m_RayCaster->GetSampleDistance(); }
+
+void PSART::Init() {}
+
+void PSART::Resize(int width, int height) {
+  Scene::Resize(width, height);
+  m_RayProducer->SetViewPort(width, height);
+}
+
+WapLoopConfig PSART::LoopConfig() const {
+  WapLoopConfig c;
+  memset(&c, 0, sizeof(c));
+  for (int a = 0; a < 3; ++a) c.dims[a] = m_voldims[a];
+  c.nviews = m_projNo;
+  c.nu = m_ViewPort[0];
+  c.nv = m_ViewPort[1];
+  c.sd = m_RayCaster->GetSampleDistance();
+  c.alpha = m_alpha;
+  c.gmax = G_Max;
+  c.nb_sart = nb_proxiSART;
+  for (int q = 0; q < 8; ++q) c.roi[q] = c.cut[q] = m_Data ? m_Data->m_roi[q] : 8;
+  c.flow_tol = flow_tol;
+  c.flow_lambda = flow_lambda;
+  c.flow_eta = flow_eta;
+  c.flow_sigma = flow_sigma;
+  c.flow_iters = flow_iters;
+  c.flow_warps = flow_warps;
+  c.flow_scales = flow_scales;
+  c.do_flow = 1;
+  c.do_warp = 1;
+  c.timeunit = 1.0f;
+  return c;
+}
+
+void PSART::SART_Proximal_Operator(int iframes, int nb_sart, int outloop, int stIter, int flevel) {
+  (void)outloop;
+  (void)stIter;
+  Frame_item& f = m_Data->m_Frameslist[flevel][iframes];
+  WapLoopConfig c = LoopConfig();
+  c.nb_sart = nb_sart;
+  c.nstates = (int)m_Data->m_Frameslist[flevel].size();
+  Ok(wap_sart_prox(m_ctx, &c, f.fVolume.data(), m_sino.data(), f.fStartproj, f.fEndproj,
+                   m_roiMask.data(), nullptr));
+}
+
+void PSART::WarpAndProjct(int iproj, int iframes, int* dimension, float rtime, float* aabb,
+                          int outloop, int warpiter, char* _file_header, int flevel) {
+  (void)dimension;
+  (void)aabb;
+  (void)outloop;
+  (void)warpiter;
+  (void)_file_header;
+  WapLoopConfig c = LoopConfig();
+  c.nstates = (int)m_Data->m_Frameslist[flevel].size();
+  Ok(wap_warp_and_project(m_ctx, &c, m_Data->m_LevelVolumes[flevel].data(),
+                          m_Data->m_LevelFlows[flevel].data(), m_sino.data(), iframes, iproj,
+                          rtime, m_roiMask.data(), nullptr));
+}
+
+// The outer loop (Project/WaP_Tomo.cpp:473-877) on the device: per temporal level, the
+// CP iterations (TV/temporal-prior step + SART proximal sweeps per frame), the flow of
+// every adjacent pair, the warp-and-project pass, then the level upsampling by copy.
+void PSART::Render() {
+  if (!m_Data) return;
+  const int n = m_voldims[0] * m_voldims[1] * m_voldims[2];
+  const float aabb[6] = {0, (float)m_voldims[0], 0, (float)m_voldims[1], 0, (float)m_voldims[2]};
+  for (int level = 0; level < nlevels; ++level) {
+    const int nf = m_Data->m_fInLevel[level];
+    DeviceImage Ytv(m_voldims[0], m_voldims[1], m_voldims[2], 3 * (nf + 3), 0.0f);
+    DeviceImage Ytp(m_voldims[0], m_voldims[1], m_voldims[2], nf + 3, 0.0f);
+    DeviceImage Xbar(m_voldims[0], m_voldims[1], m_voldims[2], nf + 3, 0.0f);
+    DeviceImage Xprev(m_voldims[0], m_voldims[1], m_voldims[2], nf + 3, 0.0f);
+    const int roi6[6] = {m_Data->m_roi[0], m_Data->m_roi[1], m_Data->m_roi[2],
+                         m_Data->m_roi[3], m_Data->m_roi[4], m_Data->m_roi[5]};
+    for (int cp = 1; cp <= nb_volOpt[level < 2 ? level : 1]; ++cp) {
+      for (int f = 0; f < nf; ++f) {
+        float* vol = m_Data->m_Frameslist[level][f].fVolume.data();
+        Ok(wap_tvtp_step(m_ctx, f, m_voldims, roi6, vol, Xprev.data() + (size_t)f * n,
+                         Xbar.data() + (size_t)f * n, Xbar.data() + (size_t)(f + 1) * n,
+                         Ytv.data() + (size_t)3 * f * n, Ytp.data() + (size_t)f * n,
+                         f > 0 ? Ytp.data() + (size_t)(f - 1) * n : nullptr, CP_sigma, CP_tau,
+                         TV_w, TP_w, G_Max, nullptr));
+        SART_Proximal_Operator(f, nb_proxiSART, 0, cp, level);
+        Ok(wap_xbar_update(m_ctx, m_voldims, roi6, Xbar.data() + (size_t)f * n, vol,
+                           Xprev.data() + (size_t)f * n, nullptr));
+      }
+    }
+    WapLoopConfig c = LoopConfig();
+    DeviceImage out(m_voldims[0], m_voldims[1], m_voldims[2], 3, 0.0f);
+    if (m_flowSet.size() < (size_t)(nlevels * 131)) m_flowSet.resize(nlevels * 131, 0);
+    for (int f = 0; f + 1 < nf; ++f) {
+      Frame_item& a = m_Data->m_Frameslist[level][f];
+      Frame_item& b = m_Data->m_Frameslist[level][f + 1];
+      const bool init = m_flowSet[level * 131 + f] != 0;
+      Ok(wap_flow_multiscale(m_ctx, m_voldims, a.fVolume.data(), b.fVolume.data(),
+                             init ? a.fFlow.data() : nullptr, c.flow_tol, c.flow_lambda,
+                             c.flow_iters, c.flow_warps, c.flow_eta, c.flow_sigma,
+                             c.flow_scales, c.cut, out.data(), nullptr));
+      a.fFlow = out;
+      m_flowSet[level * 131 + f] = 1;
+    }
+    const float trange = m_Data->m_tUnitInLevel[level];
+    for (int fidx = 0; fidx < m_Data->m_nframes; ++fidx) {
+      const Frame_item& fr = m_Data->m_Frameslist[level][fidx];
+      for (int ik = fr.fStartproj; ik <= fr.fEndproj; ++ik) {
+        float td = (ik - fr.fTime) / trange;
+        td = std::min(std::max(-1.0f, td), 1.0f);
+        int dims[3] = {m_voldims[0], m_voldims[1], m_voldims[2]};
+        WarpAndProjct(ik, fidx, dims, td, const_cast<float*>(aabb), 0, 0, nullptr, level);
+      }
+    }
+    if (level < nlevels - 1) {
+      // each state seeds the two finer states it spans (the finer level holds at most
+      // 130 states plus two tail copies, so the upper states of a long coarse level
+      // have no room)
+      std::vector<Frame_item>& fine = m_Data->m_Frameslist[level + 1];
+      for (int i = 0; i < nf; ++i)
+        for (int k = 2 * i; k <= 2 * i + 1; ++k)
+          if (k < (int)fine.size()) fine[k].fVolume = m_Data->m_Frameslist[level][i].fVolume;
+    }
+  }
+  (void)hipDeviceSynchronize();
+}

@@ -1,0 +1,296 @@
+// Drop-in C++ classes of the warp-and-project loop with the reference's names and
+// method signatures, backed by the gfx950 kernels of libwap (include/wap.h).
+//
+//   PSART (alias WaP_Tomo)  Project/WaP_Tomo.h:21-131, Project/WaP_Tomo.cpp:228-877,1207-1797
+//   Scene                   Project/Scene.h, Project/Scene.cpp:65-189 (camera + matrices only)
+//   TypicalRayProducer      Project/TypicalRayProducer.h:8-100, .cpp:27-262
+//   RayCaster               Project/RayCaster.h:9-30
+//   CoherentVolume          Project/CoherentVolume.h:10-97, .cpp:19-283
+//   VolumeData, Frame_item  Project/VolumeData.h:19-89, Project/Frames.h:16-63
+//
+// Differences, by design: images live on the device (DeviceImage instead of CImg);
+// the projector state is per object instead of file-static; there is no GL window
+// (Render() is the headless driver) and no TIFF/Analyze I/O (projections and angles
+// come from raw float32 / text files or from memory through SetProjections/SetAngles).
+#pragma once
+
+#include <stddef.h>
+
+#include <string>
+#include <vector>
+
+#include "../../../include/wap.h"
+#include "wap_math.h"
+
+// A W x H x D x C float field in device memory (planar, x fastest). Copies are deep, as
+// CImg's assignment; an image may also be a non-owning view into a larger buffer.
+class DeviceImage {
+ public:
+  DeviceImage() = default;
+  DeviceImage(int w, int h, int d, int c, float fill = 0.0f);
+  DeviceImage(const DeviceImage& o);
+  DeviceImage& operator=(const DeviceImage& o);
+  DeviceImage& operator=(float value);
+  ~DeviceImage();
+  static DeviceImage View(float* data, int w, int h, int d, int c);
+  void assign(int w, int h, int d, int c, float fill = 0.0f);
+  bool is_empty() const { return !data_ || size() == 0; }
+  size_t size() const { return (size_t)w_ * h_ * d_ * c_; }
+  int width() const { return w_; }
+  int height() const { return h_; }
+  int depth() const { return d_; }
+  int spectrum() const { return c_; }
+  float* data() const { return data_; }
+  void upload(const float* host);
+  void download(float* host) const;
+
+ private:
+  float* data_ = nullptr;
+  int w_ = 0, h_ = 0, d_ = 0, c_ = 0;
+  bool owner_ = false;
+};
+
+typedef DeviceImage image;
+
+class Frame_item {
+ public:
+  Frame_item() = default;
+  Frame_item(int volsize[3]);
+  void SetVolSize(int volsize[3]);
+  void GetVolSize(int* volsize);
+  float fTime = 0.0f;
+  image fVolume;
+  image fFlow;
+  int fprojRange = 0;
+  int fStartproj = -1;
+  int fEndproj = -1;
+  int fsize[3] = {0, 0, 0};
+};
+
+typedef struct {
+  int volsize[3];
+} size_item;
+
+class VolumeData {
+ public:
+  VolumeData();
+  VolumeData(int d1, int d2, int d3, float sp1, float sp2, float sp3, float* data);
+  virtual ~VolumeData();
+  void Clean();
+  void GetDimensions(int* dims) const;
+  void SetDimensions(int* dims);
+  void GetSpacings(float* spacings) const;
+  void SetSpacings(const float* spacings);
+  void SetProjectionMode(bool mode) { m_IsBackProjectMode = mode; }
+  bool GetProjectionMode() { return m_IsBackProjectMode; }
+  void setParas(int* vols, double vs, int nframes, int totalproj);
+  // frame lists of two temporal levels with nf and min(2nf, 130) frames, view windows
+  // [fTime-3, fTime+3], two duplicated tail frames; volumes and flows are contiguous per
+  // level on the device
+  void ReadStructuredFile();
+
+  bool m_IsBackProjectMode = false;
+  int m_Dims[3] = {0, 0, 0};
+  float m_Spacings[3] = {1.0f, 1.0f, 1.0f};
+  int m_volX = 0, m_volY = 0, m_volZ = 0;
+  double Vol_Spacing = 1.0;
+  std::vector<Frame_item> m_Frameslist[5];
+  int m_fInLevel[5] = {0, 0, 0, 0, 0};
+  float m_tUnitInLevel[5] = {0, 0, 0, 0, 0};
+  std::vector<size_item> m_Volumesize;
+  int m_nframes = 0;
+  int m_totalProjs = 0;
+  int m_roi[8] = {8, 8, 8, 8, 8, 8, 8, 8};
+  int m_levels = 2;
+  int m_base = 3;
+  // contiguous device storage of level l: volumes (nf+2)*N, flows (nf+2)*3N
+  DeviceImage m_LevelVolumes[5];
+  DeviceImage m_LevelFlows[5];
+};
+
+class RayCaster {
+ public:
+  RayCaster();
+  virtual ~RayCaster();
+  void SetSampleDistance(float sd) { m_SampleDistance = sd; }
+  float GetSampleDistance() { return m_SampleDistance; }
+  void MakeReference() { m_OldLightDirection = m_LightDirection; }
+  void RotateLightDirection(const Matrix4x4& matrix) {
+    m_LightDirection = matrix * m_OldLightDirection;
+  }
+
+ private:
+  Vector4 m_LightDirection;
+  Vector4 m_OldLightDirection;
+  float m_SampleDistance = 1.0f;
+};
+
+class TypicalRayProducer {
+ public:
+  TypicalRayProducer(int volX, int volY, int volZ, int imgW, int imgH, double ds, const char* bp);
+  virtual ~TypicalRayProducer();
+  void Initial();  // zero the image/correction (and, ray-based, the volume) buffers
+  void SetBackProjectionMode() { m_BackProjection = true; }
+  bool IsBackProjectionMode() { return m_BackProjection; }
+  void SetViewPort(int width, int height);
+  void SetVolSize(const int volDims[3], const float volSpacings);
+  void GetVolSize(int volDims[3], float volSpacings);
+  void SetViewMatrix(const Matrix4x4& ViewMatrix) { m_ViewMatrix = ViewMatrix; }
+  void SetProjectionMatrix(const Matrix4x4& ProjectionMatrix) {
+    m_ProjectionMatrix = ProjectionMatrix;
+  }
+  void UpdateMatrixAndBounds();
+  const Matrix4x4& GetPixelsToModelMatrix() { return m_PixelsToModelMatrix; }
+  const Matrix4x4& GetModelToVolumeMatrix() { return m_ModelToVolumeMatrix; }
+  void GetImageInUseSize(int imageSize[2]);
+  void GetImageOrigin(int imageOrigin[2]);
+  // device buffers (Nu*Nv images; W*H*D volumes for the ray-based backprojector)
+  float* GetTotalWeight() { return m_EachVoxel_TotalWeight; }
+  float* GetImageData() { return m_ImageData; }
+  float* GetWeightOfImage() { return m_WeightOfImage; }
+  float* GetImageRaylength() { return m_imageraylength; }
+  float* GetContributions() { return m_Voxel_Contributions; }
+  float* GetCorrection() { return m_Correction; }
+  // the per-view record the kernels take, from the current matrices
+  WapView CurrentView(float sdd, double half_w, double half_h, double ds) const;
+
+  void _calculateMatrix();
+  int m_ViewPort[2] = {0, 0};
+  int m_VolDims[3] = {0, 0, 0};
+  float m_VolSpacings[3] = {1.0f, 1.0f, 1.0f};
+  float m_VolSize[3] = {0, 0, 0};
+  Matrix4x4 m_ViewMatrix, m_ProjectionMatrix, m_PixelsToModelMatrix, m_ModelToPixelsMatrix,
+      m_ModelToVolumeMatrix, m_PixelsToViewMatrix, m_VolumeToModelMatrix, m_model_view;
+  float* m_Correction = nullptr;
+  Vector4 m_CameraPos;
+  int m_ImageInUseSize[2] = {0, 0};
+  int m_ImageOrigin[2] = {0, 0};
+  float* m_ImageData = nullptr;
+  float* m_WeightOfImage = nullptr;
+  float* m_EachVoxel_TotalWeight = nullptr;
+  float* m_Voxel_Contributions = nullptr;
+  float* m_imageraylength = nullptr;
+  bool m_BackProjection = false;
+  bool m_RayBased = false;
+};
+
+// Bricked volume with per-brick min/max and elision of uniform bricks (host storage).
+// The reference advances its source pointer by bytes on a float* (a stride that only
+// works for single-brick volumes); here strides are in elements.
+class CoherentVolume {
+ public:
+  class BlockArray;
+  CoherentVolume(int x, int y, int z);
+  virtual ~CoherentVolume();
+  virtual void Initialize();
+  void SetBlockSize(const int bs[3]) {
+    m_BlockSize[0] = bs[0];
+    m_BlockSize[1] = bs[1];
+    m_BlockSize[2] = bs[2];
+  }
+  void GetBlcokSize(int bs[3]) {
+    bs[0] = m_BlockSize[0];
+    bs[1] = m_BlockSize[1];
+    bs[2] = m_BlockSize[2];
+  }
+  void SetDimensions(const int dims[3]);
+  void GetDimensions(int dims[3]) const;
+  void SetSpacings(const float s[3]);
+  void GetSpacings(float s[3]) const;
+  void GetBlockNum(int bn[3]);
+  void AllocateBlocks();
+  void SetBlockSlice(void* sliceData, int sliceID, int dimz = -1);
+  void SetData(float* data);
+  float GetValue(int x, int y, int z);
+  void GetMinMaxValue(double& minValue, double& maxValue);
+  // number of bricks holding data (non-uniform)
+  int StoredBlocks() const;
+
+ private:
+  BlockArray* m_BlockArray = nullptr;
+  int m_BlockSize[3] = {32, 32, 32};
+  int m_Dimensions[3] = {0, 0, 0};
+  float m_Spacings[3] = {1.0f, 1.0f, 1.0f};
+  int m_BlockNum[3] = {0, 0, 0};
+};
+
+class Scene {
+ public:
+  Scene();
+  virtual ~Scene() = default;
+  const Matrix4x4& GetViewMatrix() { return m_ViewMatrix; }
+  const Matrix4x4& GetProjectionMatrix() { return m_ProjectionMatrix; }
+  virtual void Resize(int width, int height);
+  void SetSize3DModified();
+  void _calculateCamera();
+  void _calculateMatrix();
+
+  int m_ViewPort[2] = {0, 0};
+  float m_Near = 1.0f, m_Far = 5.0f;
+  float m_ZTranslateBase = 0.0f;
+  float m_Translate[3] = {0, 0, 0};
+  float m_Scale = 1.0f;
+  Quaternion m_Rotation, m_OldRotation;
+  bool m_NeedCalculateCamera = false, m_NeedCalculateMatrix = false;
+  Matrix4x4 m_ViewMatrix, m_ProjectionMatrix;
+  float m_sid = 0.0f, m_sdd = 0.0f;
+  double m_halfFrustumHeight = 0.0, m_halfFrustumWidth = 0.0, m_ds = 1.0;
+};
+
+class PSART : public Scene {
+ public:
+  PSART(double d1, double d2, double d3, const char* outputfile, int* imgwh, int projNo,
+        int iters, int sartiters, double alp, double sid, const char** projsfile, double* oxyz,
+        double sdd, double ds, const char* prior, const char* bp, int* volxyz, int nframes,
+        double* startdg, int nrounds);
+  virtual ~PSART();
+  bool RotateVolume(int i);
+  void SetData(VolumeData* data);
+  void SART_Proximal_Operator(int iframes, int nb_sart, int outloop, int stIter, int flevel);
+  void WarpAndProjct(int iproj, int iframes, int* dimension, float rtime, float* aabb,
+                     int outloop, int warpiter, char* _file_header, int flevel);
+  void SetSampleDistance(float sd);
+  float GetSampleDistance();
+  virtual void Init();
+  virtual void Resize(int width, int height);
+  virtual void Render();
+
+  // additions for a headless, file-format-free host
+  int SetProjections(const float* host_stack, int nviews);  // nviews * W * H floats
+  int SetAngles(const double* degrees, int n);
+  WapContext* Context() { return m_ctx; }
+  const WapView* Views() const { return m_views.data(); }
+  int LastError() const { return m_err; }
+  // outer-loop schedule (reference Render locals, Project/WaP_Tomo.cpp:478-492, 711-727)
+  int nb_volOpt[2] = {2, 10};
+  int nb_proxiSART = 2;
+  int flow_iters = 3000, flow_warps = 3, flow_scales = 5;
+  float flow_tol = 0.008f, flow_lambda = 0.05f, flow_eta = 0.5f, flow_sigma = 0.8f;
+  int nlevels = 2;
+  float G_Max = 10.0f;
+
+ private:
+  void BuildViews();
+  WapLoopConfig LoopConfig() const;
+  bool Ok(int r);
+
+  VolumeData* m_Data = nullptr;
+  RayCaster* m_RayCaster = nullptr;
+  TypicalRayProducer* m_RayProducer = nullptr;
+  WapContext* m_ctx = nullptr;
+  int m_projNo = 0;
+  int m_nframes = 0, m_nrounds = 0;
+  int m_voldims[3] = {0, 0, 0};
+  double m_volspace = 1.0;
+  double m_alpha = 0.1;
+  float TV_w = 0.01f, TP_w = 0.05f, CP_tau = 1.0f, CP_sigma = 0.1125f;
+  std::vector<double> m_angles;
+  std::vector<WapView> m_views;
+  DeviceImage m_sino;   // projNo * H * W
+  DeviceImage m_roiMask;
+  int m_err = 0;
+  std::string m_output;
+  std::vector<int> m_flowSet;  // per (level, pair): a previous flow exists (initial guess)
+};
+
+typedef PSART WaP_Tomo;

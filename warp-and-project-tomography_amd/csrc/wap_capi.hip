@@ -32,19 +32,17 @@ hipError_t wapk_bp_voxel(float*, const int*, const float*, const WapView*, int, 
                          float, const int*, const int*, int, hipStream_t);
 hipError_t wapk_advect(const int*, const float*, const float*, const float*, int, float, float*,
                        hipStream_t);
-size_t wapk_flow_primal_blocks(const int*, const int*);
-size_t wapk_flow_dual_blocks(const int*);
+size_t wapk_flow_blocks(int, const int*);
 hipError_t wapk_flow_prep(const int*, const int*, const float*, const float*, const float*,
                           float*, float*, hipStream_t);
 hipError_t wapk_flow_kx4(const int*, const float*, const float*, float*, hipStream_t);
-hipError_t wapk_flow_primal(const int*, const int*, const float*, float*, const float*,
-                            const float*, float*, int, int, double*, const int*, hipStream_t);
-hipError_t wapk_flow_dual(const int*, const int*, const float*, const float*, float*,
-                          const float*, const float*, const float*, int, int, float, float,
-                          double*, int*, int, hipStream_t);
+hipError_t wapk_flow_pd(int, const int*, const int*, const float*, float*, const float*, float*,
+                        const float*, const float*, const float*, float*, int, int, int, float,
+                        float, double*, double*, int*, int, hipStream_t);
 hipError_t wapk_flow_resid(const double*, int, const double*, int, int, float, int*, float*,
                            hipStream_t);
-hipError_t wapk_flow_settle(size_t, int, float*, const float*, int*, hipStream_t);
+hipError_t wapk_flow_settle(size_t, int, float*, const float*, float*, const float*,
+                            const int*, hipStream_t);
 hipError_t wapk_flow_arm(int*, hipStream_t);
 hipError_t wapk_deriche(const float*, float*, const int*, int, const float*, hipStream_t);
 hipError_t wapk_resize_avg(const float*, float*, int, int, size_t, size_t, const int*,
@@ -89,6 +87,7 @@ struct WapContext {
   int nviews = 0;
   std::map<std::string, DevBuf> bufs;
   std::map<std::pair<int, int>, ResizeTable> tables;
+  int flow_variant = 0;  // primal-dual iteration form (see wap_set_tuning)
 };
 
 namespace {
@@ -304,32 +303,32 @@ int flow_level(WapContext* c, const int* dims, const float* I1, const float* I2,
   const int lo[3] = {roi[0], roi[2], roi[4]};
   const int hi[3] = {dims[0] - roi[1], dims[1] - roi[3], dims[2] - roi[5]};
   const bool has_roi = hi[0] > lo[0] && hi[1] > lo[1] && hi[2] > lo[2];
-  float *vB, *yW, *u, *ut, *warped, *kty, *kx4;
+  float *vB, *yA, *yB, *u, *ut, *warped, *kty, *kx4;
   double *pp, *dp;
   int* ctl;
   float* err;
-  const size_t npb = std::max<size_t>(wapk_flow_primal_blocks(dims, roi), 1);
-  const size_t ndb = wapk_flow_dual_blocks(dims);
+  const size_t nblk = wapk_flow_blocks(c->flow_variant, dims);
   WAP_TRY(scratch(c, "flow_vB", 3 * n, &vB));
-  WAP_TRY(scratch(c, "flow_y", 10 * n, &yW));
+  WAP_TRY(scratch(c, "flow_yA", 10 * n, &yA));
+  WAP_TRY(scratch(c, "flow_yB", 10 * n, &yB));
   WAP_TRY(scratch(c, "flow_u", 3 * n, &u));
   WAP_TRY(scratch(c, "flow_ut", n, &ut));
   WAP_TRY(scratch(c, "flow_warped", n, &warped));
   WAP_TRY(scratch(c, "flow_kty", 3 * n, &kty));
   WAP_TRY(scratch(c, "flow_kx4", n, &kx4));
-  WAP_TRY(scratch(c, "flow_pp", npb, &pp));
-  WAP_TRY(scratch(c, "flow_dp", ndb, &dp));
+  WAP_TRY(scratch(c, "flow_pp", nblk, &pp));
+  WAP_TRY(scratch(c, "flow_dp", nblk, &dp));
   WAP_TRY(scratch(c, "flow_ctl", 4, &ctl));
   WAP_TRY(scratch(c, "flow_err", 4, &err));
-  // state: v in V (A) and vB; y evolves in yW everywhere, written back over the ROI
+  // state: v in V / vB, y in yA / yB (y evolves everywhere and is written back to Y over
+  // the ROI at the end); u and ut stay zero outside the ROI
   WAP_HIP(c, wapk_copy(vB, V, 3 * n, st));
-  WAP_HIP(c, wapk_copy(yW, Y, 10 * n, st));
+  WAP_HIP(c, wapk_copy(yA, Y, 10 * n, st));
   WAP_HIP(c, wapk_fill(u, 3 * n, 0.0f, st));
   WAP_HIP(c, wapk_fill(ut, n, 0.0f, st));
   WAP_HIP(c, wapk_fill(kty, 3 * n, 0.0f, st));
   const float sigma1 = std::min(1.0f / 3.0f, std::min(1.0f / 3.0f, 1.0f / 3.0f));
-  float* vcur = V;
-  float* vnext = vB;
+  float *vcur = V, *vnext = vB, *ycur = yA, *ynext = yB;
   int total = 0;
   std::vector<int> ctl_host(4);
   for (int w = 0; w < nwarps; ++w) {
@@ -342,21 +341,14 @@ int flow_level(WapContext* c, const int* dims, const float* I1, const float* I2,
       const int mode = (it == 1) ? (w == 0 ? 0 : 1) : 2;
       const int resid = (it % 50 == 0);
       const int store = (it % 50 == 49);
-      if (has_roi)
-        WAP_HIP(c, wapk_flow_primal(dims, roi, vcur, vnext, yW, u, kty, store, resid, pp, ctl,
-                                    st));
-      else
-        WAP_HIP(c, wapk_copy(vnext, vcur, 3 * n, st));
-      WAP_HIP(c, wapk_flow_dual(dims, roi, vcur, vnext, yW, u, ut, kx4, mode, resid, sigma1,
-                                lambda, dp, ctl, it, st));
-      if (resid) {
-        if (!has_roi) WAP_HIP(c, wapk_fill_double(pp, npb, st));
-        WAP_HIP(c, wapk_flow_resid(pp, has_roi ? (int)wapk_flow_primal_blocks(dims, roi) : 1, dp,
-                                   (int)ndb, (int)n, tol, ctl, err, st));
-      }
+      WAP_HIP(c, wapk_flow_pd(c->flow_variant, dims, roi, vcur, vnext, ycur, ynext, u, ut, kx4, kty, mode, resid,
+                              store, sigma1, lambda, pp, dp, ctl, it, st));
+      if (resid)
+        WAP_HIP(c, wapk_flow_resid(pp, (int)nblk, dp, (int)nblk, (int)n, tol, ctl, err, st));
       std::swap(vcur, vnext);
+      std::swap(ycur, ynext);
     }
-    WAP_HIP(c, wapk_flow_settle(3 * n, last, vcur, vnext, ctl, st));
+    WAP_HIP(c, wapk_flow_settle(n, last, vcur, vnext, ycur, ynext, ctl, st));
     if (iters_run) {
       WAP_HIP(c, hipMemcpyAsync(ctl_host.data(), ctl, sizeof(int) * 2, hipMemcpyDeviceToHost, st));
       WAP_HIP(c, hipStreamSynchronize(st));
@@ -364,7 +356,7 @@ int flow_level(WapContext* c, const int* dims, const float* I1, const float* I2,
     }
   }
   if (vcur != V) WAP_HIP(c, wapk_copy(V, vcur, 3 * n, st));
-  if (has_roi) WAP_HIP(c, wapk_copy_box(Y, yW, dims, 10, lo, hi, st));
+  if (has_roi) WAP_HIP(c, wapk_copy_box(Y, ycur, dims, 10, lo, hi, st));
   if (iters_run) *iters_run = total;
   return WAP_OK;
 }
@@ -511,6 +503,25 @@ int check_loop(WapContext* c, const WapLoopConfig* L) {
 extern "C" {
 
 int wap_version(void) { return 1; }
+
+int wap_set_tuning(WapContext* c, const char* key, int value) {
+  if (!c || !key) return WAP_E_ARG;
+  if (strcmp(key, "flow_variant") == 0) {
+    if (value < 0 || value > 5) return fail(c, WAP_E_ARG, "flow_variant must be 0..5");
+    c->flow_variant = value;
+    return WAP_OK;
+  }
+  return fail(c, WAP_E_ARG, std::string("unknown tuning key ") + key);
+}
+
+int wap_get_tuning(const WapContext* c, const char* key, int* value) {
+  if (!c || !key || !value) return WAP_E_ARG;
+  if (strcmp(key, "flow_variant") == 0) {
+    *value = c->flow_variant;
+    return WAP_OK;
+  }
+  return WAP_E_ARG;
+}
 
 int wap_ctx_create(WapContext** out, int device) {
   if (!out) return WAP_E_ARG;
@@ -682,6 +693,8 @@ int wap_xbar_update(WapContext* c, const int* dims, const int* roi6, float* xbar
                     const float* vol, const float* xprev, void* stream) {
   if (!c || !dims_ok(dims) || !roi6 || !xbar || !vol || !xprev)
     return fail(c, WAP_E_ARG, "wap_xbar_update: invalid arguments");
+  for (int q = 0; q < 6; ++q)
+    if (roi6[q] < 0) return fail(c, WAP_E_ARG, "wap_xbar_update: ROI margins must be >= 0");
   WAP_HIP(c, wapk_xbar(dims, roi6, xbar, vol, xprev, S(stream)));
   return WAP_OK;
 }

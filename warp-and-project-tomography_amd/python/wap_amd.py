@@ -51,6 +51,8 @@ SIGNATURES = {
     "wap_ctx_create": [C.POINTER(_vp), C.c_int],
     "wap_ctx_destroy": [_vp],
     "wap_last_error": [_vp],
+    "wap_set_tuning": [_vp, C.c_char_p, C.c_int],
+    "wap_get_tuning": [_vp, C.c_char_p, _ip],
     "wap_view_geometry": [C.POINTER(WapScanGeometry), C.c_double, C.POINTER(WapView)],
     "wap_view_order": [C.c_int, C.c_int, _ip],
     "wap_frame_windows": [C.c_int, C.c_int, C.c_float, _fp, _ip, _ip],
@@ -189,6 +191,14 @@ class Context:
         if r != WAP_OK:
             msg = self.lib.wap_last_error(self.h)
             raise RuntimeError(f"{what} failed ({r}): {msg.decode() if msg else ''}")
+
+    def set_tuning(self, key, value):
+        self._check(self.lib.wap_set_tuning(self.h, key.encode(), int(value)), "wap_set_tuning")
+
+    def get_tuning(self, key):
+        v = C.c_int(0)
+        self._check(self.lib.wap_get_tuning(self.h, key.encode(), C.byref(v)), "wap_get_tuning")
+        return v.value
 
     # -- geometry -------------------------------------------------------------------
     def upload_views(self, view_array):
