@@ -39,8 +39,15 @@ const char* wap_last_error(const WapContext* ctx);
 int wap_version(void);
 
 /* Tuning knobs (no reference counterpart; results are identical for every value):
- *   "flow_variant": 0 = two-kernel primal/dual iteration, 1..5 = fused z-marching
- *   iteration with tiles (64,4,2) (64,4,4) (64,4,8) (32,8,4) (64,4,16). */
+ *   "flow_variant": form of the optical-flow primal-dual iteration. -1 (default) = the
+ *   built-in autotuner times the candidate forms on the first flow level of each grid
+ *   size and keeps the fastest; 0 = two kernels (primal over the ROI, then dual over the
+ *   grid); 1..5 = one kernel with LDS plane tiles; 6..29 = one single-pass kernel whose
+ *   threads march z over column tiles (tile and chunk shapes listed in flow.hip).
+ *   "flow_retune" (set, any value): forget the autotuner's choices.
+ *   get-only: "flow_variant_used" (form of the last flow level), "flow_variant_count".
+ * The autotuner synchronises the stream once per new grid size; callers that capture
+ * a graph run one warm-up call first or set a fixed form. */
 int wap_set_tuning(WapContext* ctx, const char* key, int value);
 int wap_get_tuning(const WapContext* ctx, const char* key, int* value);
 

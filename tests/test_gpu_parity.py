@@ -249,14 +249,15 @@ def test_wap_iteration(wctx, oracle, W, parallel):
     assert rv.max() > 0.1
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4, 5])
+@pytest.mark.parametrize("variant", list(range(30)))
 def test_flow_iteration_variants_identical(wctx, oracle, variant):
     """Every primal-dual iteration form (two kernels, fused tiles) gives the oracle's
     floats, on a wide grid (64-column tiles) and a narrow one (16x16 tiles)."""
     prev = wctx.get_tuning("flow_variant")
     wctx.set_tuning("flow_variant", variant)
     try:
-        for shape, roi in [((20, 18, 70), [3, 2, 1, 2, 2, 3]), ((14, 16, 18), [2] * 6)]:
+        for shape, roi in [((20, 18, 70), [3, 2, 1, 2, 2, 3]), ((14, 16, 18), [2] * 6),
+                           ((37, 11, 67), [1, 0, 2, 1, 0, 3])]:
             I1 = oracle.blur(helpers.blob_volume(shape, 12), 1.0)
             I2 = oracle.advect(np.full((3,) + shape, 0.4, np.float32), I1, 1.0)
             z3 = np.zeros((3,) + shape, np.float32)

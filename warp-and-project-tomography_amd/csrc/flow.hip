@@ -17,6 +17,8 @@
 // sigma2 are recomputed from u. The run/stop decision lives on the device (ctl[0]) so a
 // whole warp of iterations can be queued (or captured in a graph) without host round
 // trips.
+#include <algorithm>
+
 #include "wap_device.h"
 
 namespace {
@@ -326,6 +328,257 @@ __global__ __launch_bounds__(256) void k_flow_dual(
   }
 }
 
+// Block sum over a workgroup of up to 1024 threads (linear id x-fastest).
+WAP_DEV double block_sum_any(double v, int tid, int nthreads) {
+  __shared__ double red[16];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  if ((tid & 63) == 0) red[tid >> 6] = v;
+  __syncthreads();
+  double s = 0.0;
+  if (tid == 0)
+    for (int w = 0; w < (nthreads + 63) / 64; ++w) s += red[w];
+  return s;
+}
+
+// One point of the z-marching iteration: its old v, the new v of the primal step, and
+// (FULL) every old dual and u at the point, which the dual step reuses. c3 holds y_c3
+// (planes 2, 5, 8) at (i, j, k-1) on entry and at (i, j, k) on return.
+template <bool FULL>
+struct PdPoint {
+  float vo[3], vn[3], K[3], tau[3];
+  float y[FULL ? 10 : 1], u[FULL ? 3 : 1];
+};
+
+// Loads and stores at a 32-bit byte offset from a plane base: the base is uniform, so
+// the access takes a scalar base and one shared vector offset (no per-lane 64-bit
+// address arithmetic); neighbours at +-1 voxel fold into the immediate offset.
+WAP_DEV float ldb(const float* __restrict__ base, unsigned off) {
+  return *(const float*)((const char*)base + off);
+}
+WAP_DEV void stb(float* __restrict__ base, unsigned off, float v) {
+  *(float*)((char*)base + off) = v;
+}
+
+// Every load is unconditional, at an in-bounds (clamped) address, and the boundary and
+// ROI rules pick among the loaded values with selects: a load inside a divergent branch
+// would make the wave wait for it before leaving the branch, serialising the step.
+template <bool FULL>
+WAP_DEV void pd_point(const FlowGrid& G, const float* __restrict__ vo,
+                      const float* __restrict__ yo, const float* __restrict__ u, size_t n,
+                      int i, int j, int k, unsigned tb, bool roi, float* c3, PdPoint<FULL>& p) {
+  const unsigned syb = 4u * (unsigned)G.W;
+  const unsigned xm = i > 0 ? tb - 4u : tb, ym = j > 0 ? tb - syb : tb;
+  float y1[3], y2[3], y3[3], y1m[3], y2m[3], uu[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    p.vo[c] = ldb(vo + c * n, tb);
+    y1[c] = ldb(yo + (3 * c) * n, tb);
+    y2[c] = ldb(yo + (3 * c + 1) * n, tb);
+    y3[c] = ldb(yo + (3 * c + 2) * n, tb);
+    y1m[c] = ldb(yo + (3 * c) * n, xm);
+    y2m[c] = ldb(yo + (3 * c + 1) * n, ym);
+    uu[c] = ldb(u + c * n, tb);
+  }
+  const float y4 = ldb(yo + 9 * n, tb);
+  if constexpr (FULL) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      p.y[3 * c] = y1[c];
+      p.y[3 * c + 1] = y2[c];
+      p.y[3 * c + 2] = y3[c];
+      p.u[c] = uu[c];
+    }
+    p.y[9] = y4;
+  }
+  const bool xin = i > 0 && i < G.W - 1, yin = j > 0 && j < G.H - 1, zin = k > 0 && k < G.D - 1;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const float dx = xin ? y1[c] - y1m[c] : (i == 0 ? y1[c] : -y1m[c]);
+    const float dy = yin ? y2[c] - y2m[c] : (j == 0 ? y2[c] : -y2m[c]);
+    const float dz = zin ? y3[c] - c3[c] : (k == 0 ? y3[c] : -c3[c]);
+    const float kt = -1.0f * dx - 1.0f * dy - 1.0f * dz + uu[c] * y4;
+    const float ta = 1.0f / (4.0f / 1.0f + my_abs(uu[c]));
+    p.K[c] = roi ? kt : 0.0f;
+    p.tau[c] = roi ? ta : 1.0f;
+    p.vn[c] = roi ? p.vo[c] - ta * kt : p.vo[c];
+    c3[c] = y3[c];
+  }
+}
+
+// One whole primal-dual iteration in one pass. Each thread owns an (x, y) column and
+// marches z over a chunk of zc planes. At plane k the dual step needs the new (and old)
+// v at the point and at its +x, +y and +z neighbours: the +z one is the thread's own
+// primal step of plane k+1, computed one plane ahead and carried in registers; the +y
+// one is recomputed from the old state, or (YLDS) read through LDS from the thread of
+// the next row except in the workgroup's last row; the +x one is recomputed too, or
+// (XLDS, when the workgroup spans whole rows) read from the next thread through LDS. The
+// old duals at the -z neighbour come from the previous plane's registers. HBM traffic
+// per voxel: read v (3), y (10), u (3), ut (1); write v (3), y (10). Every load of a
+// plane step is issued before the step's stores. Threads past the grid edge run the
+// march on clamped coordinates (so every barrier is reached) and store nothing.
+// KTY: this iteration keeps K^T y (store_kty) or sums the residual terms (resid).
+template <bool KTY, bool XLDS, bool YLDS>
+__global__ __launch_bounds__(1024) void k_flow_zm(
+    FlowGrid G, const float* __restrict__ vo, float* __restrict__ vn,
+    const float* __restrict__ yo, float* __restrict__ yn, const float* __restrict__ u,
+    const float* __restrict__ ut, const float* __restrict__ kx4prev, float* __restrict__ kty,
+    int zc, int mode, int resid, int store_kty, float sigma1, float lambda,
+    double* __restrict__ ppart, double* __restrict__ dpart, int* __restrict__ ctl, int iter) {
+  extern __shared__ float xsh[];  // XLDS: 2 x 6 x (threads) floats
+  if (ctl[0] == 0) return;
+  const int W = G.W, H = G.H, D = G.D;
+  const size_t n = (size_t)W * H * D;
+  const int bx = (int)blockDim.x, by = (int)blockDim.y;
+  const int tilesX = (W + bx - 1) / bx, tilesY = (H + by - 1) / by;
+  const unsigned bid = xcd_remap(blockIdx.x, gridDim.x);
+  const int x0 = (int)(bid % (unsigned)tilesX) * bx;
+  const int y0 = (int)((bid / (unsigned)tilesX) % (unsigned)tilesY) * by;
+  const int z0 = (int)(bid / ((unsigned)tilesX * (unsigned)tilesY)) * zc;
+  const int z1 = min(z0 + zc, D);
+  const int tid = (int)(threadIdx.x + threadIdx.y * blockDim.x);
+  const int nt = bx * by;
+  if (blockIdx.x == 0 && tid == 0) ctl[1] = iter;
+  const bool act = x0 + (int)threadIdx.x < W && y0 + (int)threadIdx.y < H;
+  const int i = min(x0 + (int)threadIdx.x, W - 1), j = min(y0 + (int)threadIdx.y, H - 1);
+  double pterm = 0.0, dterm = 0.0;
+  const unsigned sy = (unsigned)W, sz = (unsigned)W * (unsigned)H;
+  const bool hx = i < W - 1, hy = j < H - 1;
+  const bool rx0 = i >= G.r0 && i < W - G.r1, ry0 = j >= G.r2 && j < H - G.r3;
+  const bool rx1 = i + 1 >= G.r0 && i + 1 < W - G.r1, ry1 = j + 1 >= G.r2 && j + 1 < H - G.r3;
+  const bool roi_o = rx0 && ry0, roi_x = hx && rx1 && ry0, roi_y = hy && rx0 && ry1;
+  const float shrink_den = lambda + 2.0f * sigma1;
+  // byte offset of (i, j, z0) in a plane (planes hold fewer than 2^30 voxels)
+  unsigned t = 4u * ((unsigned)i + sy * (unsigned)j + sz * (unsigned)z0);
+  const unsigned syb = 4u * sy, szb = 4u * sz;
+  float c3o[3] = {0.0f, 0.0f, 0.0f}, c3x[3] = {0.0f, 0.0f, 0.0f}, c3y[3] = {0.0f, 0.0f, 0.0f};
+  // +x / +y neighbours; at the grid edge they alias the point itself (values unused)
+  const int ix = hx ? i + 1 : i, jy = hy ? j + 1 : j;
+  const unsigned dxb = hx ? 4u : 0u, dyb = hy ? syb : 0u;
+  // YLDS: only the workgroup's last row (whole waves, as rows are multiples of 64
+  // threads) recomputes its +y neighbour
+  const bool ylast = (int)threadIdx.y == by - 1;
+  if (z0 > 0) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const float* y3 = yo + (3 * c + 2) * n;
+      c3o[c] = ldb(y3, t - szb);
+      if (!XLDS) c3x[c] = ldb(y3, t + dxb - szb);
+      if (!YLDS || ylast) c3y[c] = ldb(y3, t + dyb - szb);
+    }
+  }
+  auto rz = [&](int k) { return k >= G.r4 && k < D - G.r5; };
+  PdPoint<true> o;
+  PdPoint<false> nx, ny;
+  pd_point<true>(G, vo, yo, u, n, i, j, z0, t, roi_o && rz(z0), c3o, o);
+  if (!XLDS) pd_point<false>(G, vo, yo, u, n, ix, j, z0, t + dxb, roi_x && rz(z0), c3x, nx);
+  if (!YLDS || ylast)
+    pd_point<false>(G, vo, yo, u, n, i, jy, z0, t + dyb, roi_y && rz(z0), c3y, ny);
+  for (int k = z0; k < z1; ++k) {
+    const bool hz = k < D - 1;
+    const bool more = k + 1 < z1;
+    // every load of this step: own point at k+1 (the +z neighbour), the +x/+y points at
+    // k+1 for the next step, and ut / previous u.v at k; past the last plane the own
+    // point aliases plane k (values unused)
+    const int kn = hz ? k + 1 : k;
+    const unsigned tn = hz ? t + szb : t;
+    PdPoint<true> on;
+    PdPoint<false> nx1, ny1;
+    pd_point<true>(G, vo, yo, u, n, i, j, kn, tn, roi_o && rz(kn), c3o, on);
+    if (!XLDS)
+      pd_point<false>(G, vo, yo, u, n, ix, j, kn, tn + dxb, roi_x && rz(kn), c3x, nx1);
+    if (!YLDS || ylast)
+      pd_point<false>(G, vo, yo, u, n, i, jy, kn, tn + dyb, roi_y && rz(kn), c3y, ny1);
+    const float utt = ldb(ut, t);
+    const float k4p = ldb(kx4prev, t);
+    const float ktyo[3] = {KTY && resid ? ldb(kty, t) : 0.0f,
+                           KTY && resid ? ldb(kty + n, t) : 0.0f,
+                           KTY && resid ? ldb(kty + 2 * n, t) : 0.0f};
+    if constexpr (XLDS) {
+      // the +x neighbour's point of plane k is that thread's own point
+      float* sl = xsh + (k & 1) * 6 * nt;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        sl[c * nt + tid] = o.vn[c];
+        sl[(3 + c) * nt + tid] = o.vo[c];
+      }
+      __syncthreads();
+      const int tx1 = hx ? tid + 1 : tid;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        nx.vn[c] = sl[c * nt + tx1];
+        nx.vo[c] = sl[(3 + c) * nt + tx1];
+      }
+      if (YLDS && !ylast) {
+        const int ty1 = hy ? tid + bx : tid;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          ny.vn[c] = sl[c * nt + ty1];
+          ny.vo[c] = sl[(3 + c) * nt + ty1];
+        }
+      }
+    }
+    const bool has[3] = {hx, hy, hz};
+    float dt = 0.0f;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const float nbn[3] = {nx.vn[c], ny.vn[c], on.vn[c]};
+      const float nbo[3] = {nx.vo[c], ny.vo[c], on.vo[c]};
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        const float kx = has[a] ? 1.0f * (nbn[a] - o.vn[c]) : 0.0f;
+        const float ko = mode == 0 ? 0.0f : (has[a] ? 1.0f * (nbo[a] - o.vo[c]) : 0.0f);
+        const float yv = o.y[3 * c + a];
+        const float yt = yv + sigma1 * (kx + kx - ko);
+        const float ynv = yt * lambda / shrink_den;
+        if (act) stb(yn + (3 * c + a) * n, t, ynv);
+        if (KTY) dt += fabsf((yv - ynv) / sigma1 - ko + kx);
+      }
+      if (act) stb(vn + c * n, t, o.vn[c]);
+    }
+    const float ux = o.u[0], uy = o.u[1], uz = o.u[2];
+    const float k4 = ux * o.vn[0] + uy * o.vn[1] + uz * o.vn[2];
+    const float k4o =
+        mode == 0 ? 0.0f : (mode == 1 ? k4p : ux * o.vo[0] + uy * o.vo[1] + uz * o.vo[2]);
+    const float s2 = 1.0f / (fabsf(ux) + fabsf(uy) + fabsf(uz));
+    const float y4o = o.y[9];
+    const float y4t = y4o + s2 * (k4 + k4 - k4o + utt);
+    const float m1 = (y4t < 1.0f) ? y4t : 1.0f;
+    const float y4n = (-1.0f < m1) ? m1 : -1.0f;
+    if (act) stb(yn + 9 * n, t, y4n);
+    if (KTY && act) {
+      if (resid) {
+        dt += fabsf((y4o - y4n) / s2 - k4o + k4);
+        dterm += (double)dt;
+      }
+      if (roi_o && rz(k)) {
+        float pt = 0.0f;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          if (resid) pt += fabsf((o.vo[c] - o.vn[c]) / o.tau[c] - ktyo[c] + o.K[c]);
+          if (store_kty) stb(kty + c * n, t, o.K[c]);
+        }
+        if (resid) pterm += (double)pt;
+      }
+    }
+    if (more) {
+      o = on;
+      if (!XLDS) nx = nx1;
+      if (!YLDS || ylast) ny = ny1;
+    }
+    t += szb;
+  }
+  if (KTY && resid) {
+    const double ps = block_sum_any(pterm, tid, nt);
+    __syncthreads();
+    const double ds = block_sum_any(dterm, tid, nt);
+    if (tid == 0) {
+      ppart[blockIdx.x] = ps;
+      dpart[blockIdx.x] = ds;
+    }
+  }
+}
+
 // err = (d*d + p*p)/nPx in float from the float-rounded sums; stop when !(err > tol)
 __global__ __launch_bounds__(256) void k_flow_resid(const double* __restrict__ ppart, int np,
                                                     const double* __restrict__ dpart, int nd,
@@ -412,8 +665,55 @@ static PdShape shape_of(int variant, const int* dims) {
   return p;
 }
 
+// Variants 6..: the single-pass z-marching kernel (k_flow_zm) with (x, y) column tiles
+// of bx x by threads marching z chunks of zc planes. Grids narrower than bx use the
+// largest power of two <= W (at least 16) and keep the thread count.
+struct ZmShape {
+  int bx, by, zc;
+  bool xlds;  // the workgroup spans whole rows; +x neighbours through LDS
+  bool ylds;  // and +y neighbours through LDS (except in its last row)
+};
+static const ZmShape kZm[] = {
+    {64, 4, 16, false, false},  {64, 16, 16, false, false}, {128, 8, 16, false, false},
+    {256, 4, 16, false, false}, {256, 2, 16, false, false}, {256, 4, 8, false, false},
+    {128, 8, 32, false, false}, {256, 4, 32, false, false}, {64, 16, 8, false, false},
+    {256, 1, 16, false, false}, {128, 4, 16, false, false}, {128, 2, 32, false, false},
+    {256, 1, 16, true, false},  {256, 1, 32, true, false},  {256, 2, 16, true, false},
+    {256, 4, 16, true, false},  {256, 1, 8, true, false},   {256, 1, 64, true, false},
+    {256, 2, 16, true, true},   {256, 4, 16, true, true},   {256, 2, 8, true, true},
+    {256, 4, 8, true, true},    {256, 2, 32, true, true},   {256, 4, 32, true, true}};
+static const int kZmCount = (int)(sizeof(kZm) / sizeof(kZm[0]));
+
+static ZmShape zm_shape(int variant, const int* dims) {
+  ZmShape z = kZm[std::min(std::max(variant - 6, 0), kZmCount - 1)];
+  const int threads = z.bx * z.by;
+  if (z.xlds) {
+    // whole rows: the narrowest multiple of 64 that holds W (at most 1024 threads);
+    // wider grids fall back to recomputing the +x neighbour
+    const int bx = (dims[0] + 63) / 64 * 64;
+    if (bx > 1024) {
+      z.xlds = false;
+      z.ylds = false;
+    } else {
+      z.bx = bx;
+      z.by = std::max(1, std::min(threads, 1024) / bx);
+      return z;
+    }
+  }
+  while (z.bx > 16 && z.bx > dims[0]) z.bx >>= 1;
+  z.by = threads / z.bx;
+  return z;
+}
+
+int wapk_flow_variants() { return 6 + kZmCount; }
+
 size_t wapk_flow_blocks(int variant, const int* dims) {
   if (variant == 0) return ((size_t)dims[0] * dims[1] * dims[2] + 255) / 256;
+  if (variant >= 6) {
+    const ZmShape z = zm_shape(variant, dims);
+    return (size_t)((dims[0] + z.bx - 1) / z.bx) * ((dims[1] + z.by - 1) / z.by) *
+           ((dims[2] + z.zc - 1) / z.zc);
+  }
   const PdShape p = shape_of(variant, dims);
   return (size_t)((dims[0] + p.tx - 1) / p.tx) * ((dims[1] + p.ty - 1) / p.ty) *
          ((dims[2] + p.zc - 1) / p.zc);
@@ -462,6 +762,30 @@ hipError_t wapk_flow_pd(int variant, const int* dims, const int* roi, const floa
     }
     hipLaunchKernelGGL(k_flow_dual, dim3(nb), dim3(256), 0, s, G, vold, vnew, yold, u, ut,
                        kx4prev, mode, resid, sigma1, lambda, dpart, ctl, iter, ynew);
+    return hipGetLastError();
+  }
+  if (variant >= 6) {
+    const size_t n = (size_t)dims[0] * dims[1] * dims[2];
+    if (n >= (1ull << 30)) return hipErrorInvalidValue;  // 32-bit in-plane byte offsets
+    const ZmShape z = zm_shape(variant, dims);
+    const dim3 b(z.bx, z.by);
+    const int zc = z.zc;
+    const int r = resid, st = store_kty;
+#define WAP_ZM(KT, XL, YL, R, ST)                                                          \
+  hipLaunchKernelGGL((k_flow_zm<KT, XL, YL>), dim3(nb), b, XL ? lds : 0, s, G, vold, vnew, yold, \
+                     ynew, u, ut, kx4prev, kty, zc, mode, R, ST, sigma1, lambda, ppart, dpart,   \
+                     ctl, iter)
+    const size_t lds = sizeof(float) * 2 * 6 * z.bx * z.by;
+    if (r || st) {
+      if (z.ylds) WAP_ZM(true, true, true, r, st);
+      else if (z.xlds) WAP_ZM(true, true, false, r, st);
+      else WAP_ZM(true, false, false, r, st);
+    } else {
+      if (z.ylds) WAP_ZM(false, true, true, 0, 0);
+      else if (z.xlds) WAP_ZM(false, true, false, 0, 0);
+      else WAP_ZM(false, false, false, 0, 0);
+    }
+#undef WAP_ZM
     return hipGetLastError();
   }
   const PdShape p = shape_of(variant, dims);

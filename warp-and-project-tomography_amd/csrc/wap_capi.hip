@@ -13,6 +13,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <array>
 #include <map>
 #include <string>
 #include <vector>
@@ -32,6 +33,7 @@ hipError_t wapk_bp_voxel(float*, const int*, const float*, const WapView*, int, 
                          float, const int*, const int*, int, hipStream_t);
 hipError_t wapk_advect(const int*, const float*, const float*, const float*, int, float, float*,
                        hipStream_t);
+int wapk_flow_variants();
 size_t wapk_flow_blocks(int, const int*);
 hipError_t wapk_flow_prep(const int*, const int*, const float*, const float*, const float*,
                           float*, float*, hipStream_t);
@@ -87,7 +89,9 @@ struct WapContext {
   int nviews = 0;
   std::map<std::string, DevBuf> bufs;
   std::map<std::pair<int, int>, ResizeTable> tables;
-  int flow_variant = 0;  // primal-dual iteration form (see wap_set_tuning)
+  int flow_variant = -1;  // primal-dual iteration form; -1 = autotuned per grid size
+  std::map<std::array<int, 3>, int> flow_tuned;  // autotuner choices by (W, H, D)
+  int flow_last = 0;                              // form used by the last flow level
 };
 
 namespace {
@@ -296,6 +300,56 @@ int blur_impl(WapContext* c, const float* src, const int* dims, int nc, float si
 // ---------------------------------------------------------------------------------
 // optical flow: one scale (the warp/iteration loop)
 // ---------------------------------------------------------------------------------
+// Autotuner of the primal-dual iteration form: on the first flow level of a grid size,
+// time each candidate form (all give the same floats) on the level's own buffers and
+// keep the fastest for that size. The timed launches read (vcur, ycur) and write
+// (vnext, ynext), which the level then overwrites, so the state is untouched.
+int flow_tune(WapContext* c, const int* dims, const int* roi, const float* vcur, float* vnext,
+              const float* ycur, float* ynext, const float* u, const float* ut,
+              const float* kx4, float* kty, int* ctl, hipStream_t st, int* chosen) {
+  const std::array<int, 3> key = {dims[0], dims[1], dims[2]};
+  auto it = c->flow_tuned.find(key);
+  if (it != c->flow_tuned.end()) {
+    *chosen = it->second;
+    return WAP_OK;
+  }
+  static const int cand[] = {0, 15, 18, 20, 22, 24, 25, 26, 27, 28};
+  hipEvent_t e0, e1;
+  WAP_HIP(c, hipEventCreate(&e0));
+  WAP_HIP(c, hipEventCreate(&e1));
+  int best = 0;
+  float best_ms = 1e30f;
+  int rc = WAP_OK;
+  for (int v : cand) {
+    if (v >= wapk_flow_variants()) continue;
+    float ms = 0.0f;
+    hipError_t e = wapk_flow_arm(ctl, st);
+    for (int r = 0; r < 4 && e == hipSuccess; ++r) {
+      if (r == 1) e = hipEventRecord(e0, st);
+      if (e == hipSuccess)
+        e = wapk_flow_pd(v, dims, roi, vcur, vnext, ycur, ynext, u, ut, kx4, kty, 2, 0, 0,
+                         1.0f / 3.0f, 0.05f, nullptr, nullptr, ctl, 0, st);
+    }
+    if (e == hipSuccess) e = hipEventRecord(e1, st);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    if (e != hipSuccess) {
+      rc = fail(c, WAP_E_HIP, std::string("flow autotuner: ") + hipGetErrorString(e));
+      break;
+    }
+    if (ms < best_ms) {
+      best_ms = ms;
+      best = v;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc != WAP_OK) return rc;
+  c->flow_tuned[key] = best;
+  *chosen = best;
+  return WAP_OK;
+}
+
 int flow_level(WapContext* c, const int* dims, const float* I1, const float* I2, float* V,
                float* Y, float tol, float lambda, int max_iter, int nwarps, const float* aabb,
                const int* roi, int* iters_run, hipStream_t st) {
@@ -307,7 +361,6 @@ int flow_level(WapContext* c, const int* dims, const float* I1, const float* I2,
   double *pp, *dp;
   int* ctl;
   float* err;
-  const size_t nblk = wapk_flow_blocks(c->flow_variant, dims);
   WAP_TRY(scratch(c, "flow_vB", 3 * n, &vB));
   WAP_TRY(scratch(c, "flow_yA", 10 * n, &yA));
   WAP_TRY(scratch(c, "flow_yB", 10 * n, &yB));
@@ -316,8 +369,6 @@ int flow_level(WapContext* c, const int* dims, const float* I1, const float* I2,
   WAP_TRY(scratch(c, "flow_warped", n, &warped));
   WAP_TRY(scratch(c, "flow_kty", 3 * n, &kty));
   WAP_TRY(scratch(c, "flow_kx4", n, &kx4));
-  WAP_TRY(scratch(c, "flow_pp", nblk, &pp));
-  WAP_TRY(scratch(c, "flow_dp", nblk, &dp));
   WAP_TRY(scratch(c, "flow_ctl", 4, &ctl));
   WAP_TRY(scratch(c, "flow_err", 4, &err));
   // state: v in V / vB, y in yA / yB (y evolves everywhere and is written back to Y over
@@ -327,6 +378,14 @@ int flow_level(WapContext* c, const int* dims, const float* I1, const float* I2,
   WAP_HIP(c, wapk_fill(u, 3 * n, 0.0f, st));
   WAP_HIP(c, wapk_fill(ut, n, 0.0f, st));
   WAP_HIP(c, wapk_fill(kty, 3 * n, 0.0f, st));
+  WAP_HIP(c, wapk_fill(kx4, n, 0.0f, st));
+  int variant = c->flow_variant;
+  if (variant < 0)
+    WAP_TRY(flow_tune(c, dims, roi, V, vB, yA, yB, u, ut, kx4, kty, ctl, st, &variant));
+  c->flow_last = variant;
+  const size_t nblk = wapk_flow_blocks(variant, dims);
+  WAP_TRY(scratch(c, "flow_pp", nblk, &pp));
+  WAP_TRY(scratch(c, "flow_dp", nblk, &dp));
   const float sigma1 = std::min(1.0f / 3.0f, std::min(1.0f / 3.0f, 1.0f / 3.0f));
   float *vcur = V, *vnext = vB, *ycur = yA, *ynext = yB;
   int total = 0;
@@ -341,8 +400,8 @@ int flow_level(WapContext* c, const int* dims, const float* I1, const float* I2,
       const int mode = (it == 1) ? (w == 0 ? 0 : 1) : 2;
       const int resid = (it % 50 == 0);
       const int store = (it % 50 == 49);
-      WAP_HIP(c, wapk_flow_pd(c->flow_variant, dims, roi, vcur, vnext, ycur, ynext, u, ut, kx4, kty, mode, resid,
-                              store, sigma1, lambda, pp, dp, ctl, it, st));
+      WAP_HIP(c, wapk_flow_pd(variant, dims, roi, vcur, vnext, ycur, ynext, u, ut, kx4, kty,
+                              mode, resid, store, sigma1, lambda, pp, dp, ctl, it, st));
       if (resid)
         WAP_HIP(c, wapk_flow_resid(pp, (int)nblk, dp, (int)nblk, (int)n, tol, ctl, err, st));
       std::swap(vcur, vnext);
@@ -507,8 +566,13 @@ int wap_version(void) { return 1; }
 int wap_set_tuning(WapContext* c, const char* key, int value) {
   if (!c || !key) return WAP_E_ARG;
   if (strcmp(key, "flow_variant") == 0) {
-    if (value < 0 || value > 5) return fail(c, WAP_E_ARG, "flow_variant must be 0..5");
+    if (value < -1 || value >= wapk_flow_variants())
+      return fail(c, WAP_E_ARG, "flow_variant out of range");
     c->flow_variant = value;
+    return WAP_OK;
+  }
+  if (strcmp(key, "flow_retune") == 0) {
+    c->flow_tuned.clear();
     return WAP_OK;
   }
   return fail(c, WAP_E_ARG, std::string("unknown tuning key ") + key);
@@ -518,6 +582,14 @@ int wap_get_tuning(const WapContext* c, const char* key, int* value) {
   if (!c || !key || !value) return WAP_E_ARG;
   if (strcmp(key, "flow_variant") == 0) {
     *value = c->flow_variant;
+    return WAP_OK;
+  }
+  if (strcmp(key, "flow_variant_used") == 0) {
+    *value = c->flow_last;
+    return WAP_OK;
+  }
+  if (strcmp(key, "flow_variant_count") == 0) {
+    *value = wapk_flow_variants();
     return WAP_OK;
   }
   return WAP_E_ARG;
