@@ -105,6 +105,45 @@ def test_ray_and_voxel_geometry_agree(oracle, parallel):
         assert abs(pu - k % nu) < 2e-3 and abs(pv - k // nu) < 2e-3
 
 
+# ---------------------------------------------------------------- camera chain
+@pytest.mark.parametrize("parallel", [False, True])
+@pytest.mark.parametrize("angle", [0.0, 33.0, 90.0, 211.7])
+def test_view_geometry_closed_form(oracle, parallel, angle):
+    """The per-view rays follow from R_y(theta) alone, independent of the matrix chain:
+    the view frame is the volume frame rotated by theta about +y and moved back by sid
+    (theta = deg * 3.1415926/180, as the reference's RotateVolume); in view coordinates
+    the source sits at the origin and pixel (i, j) at ((i - cu) ds, (j - cv) ds, -sdd)
+    (cone) or its rays leave ((i - cu) ds, (j - cv) ds, 0) along -z (parallel)."""
+    shape, nu, nv, ds, sid, sdd = (20, 22, 24), 31, 27, 1.3, 96.0, 144.0
+    d, h, w = shape
+    g = pyoracle.make_geometry((w, h, d), nu, nv, sid=sid, sdd=sdd, ds=ds, parallel=parallel)
+    v = oracle.view(g, angle)
+    th = angle * 3.1415926 / 180.0
+    c, s = math.cos(th), math.sin(th)
+    Rt = np.array([[c, 0, -s], [0, 1, 0], [s, 0, c]])  # R_y(theta)^T: view -> volume
+    centre = np.array([(w - 1) / 2.0, (h - 1) / 2.0, (d - 1) / 2.0])
+    to_vol = lambda e: centre + Rt @ (np.asarray(e, np.float64) + np.array([0.0, 0.0, sid]))
+    z, u, tn, tf = oracle.produce_rays(shape, v, nu, nv)
+    cu, cv = (nu - 1) / 2.0, (nv - 1) / 2.0
+    for k in range(0, nu * nv, 7):
+        i, j = k % nu, k // nu
+        org = z[k, :3].astype(np.float64)
+        dirv = u[k, :3].astype(np.float64)
+        assert abs(np.linalg.norm(dirv) - 1.0) < 1e-5
+        if parallel:
+            expect_org = to_vol(((i - cu) * ds, (j - cv) * ds, 0.0))
+            expect_dir = Rt @ np.array([0.0, 0.0, -1.0])
+            # the origin lies on the ray through the pixel's source-plane point
+            off = org - expect_org
+            assert np.linalg.norm(off - np.dot(off, expect_dir) * expect_dir) < 2e-3
+        else:
+            expect_org = to_vol((0.0, 0.0, 0.0))
+            pix = to_vol(((i - cu) * ds, (j - cv) * ds, -sdd))
+            expect_dir = (pix - expect_org) / np.linalg.norm(pix - expect_org)
+            np.testing.assert_allclose(org, expect_org, atol=2e-3)
+        np.testing.assert_allclose(dirv, expect_dir, atol=2e-5)
+
+
 # ---------------------------------------------------------------- projector invariants
 @pytest.mark.parametrize("parallel", [False, True])
 def test_constant_volume_integral(oracle, parallel):
@@ -306,3 +345,20 @@ def test_degenerate_rays_terminate_with_bounded_samples(oracle):
         assert hit.any()
         assert np.all(w[hit] <= np.floor(rl[hit] / SD) + 8)
         assert rl.max() < 24 * 1.01
+
+
+def test_xbar_update_brute_force(oracle):
+    """X_bar = 2x - x_prev inside the ROI box, untouched outside it."""
+    shape = (7, 8, 9)
+    rng = np.random.default_rng(8)
+    x, xp, xb = (rng.random(shape).astype(np.float32) for _ in range(3))
+    roi = np.array([1, 2, 0, 1, 2, 1], np.int32)
+    out = xb.copy()
+    oracle.lib.orc_xbar_update_f32(pyoracle._dims(shape), roi, x, xp, out)
+    expect = xb.copy()
+    d, h, w = shape
+    for k in range(roi[4], d - roi[5]):
+        for j in range(roi[2], h - roi[3]):
+            for i in range(roi[0], w - roi[1]):
+                expect[k, j, i] = np.float32(2) * x[k, j, i] - xp[k, j, i]
+    np.testing.assert_array_equal(out, expect)
