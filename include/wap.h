@@ -45,6 +45,8 @@ int wap_version(void);
  *   grid); 1..5 = one kernel with LDS plane tiles; 6..29 = one single-pass kernel whose
  *   threads march z over column tiles (tile and chunk shapes listed in flow.hip).
  *   "flow_retune" (set, any value): forget the autotuner's choices.
+ *   "project_unroll": ray samples per trip of the forward projector's march (1, 2, 3,
+ *   4, 6, 8; default 2).
  *   get-only: "flow_variant_used" (form of the last flow level), "flow_variant_count".
  * The autotuner synchronises the stream once per new grid size; callers that capture
  * a graph run one warm-up call first or set a fixed form. */

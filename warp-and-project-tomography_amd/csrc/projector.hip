@@ -186,6 +186,137 @@ WAP_DEV float trilinear(const float* __restrict__ vol, float p0, float p1, float
   return a * g1 + cc * g2;
 }
 
+// The same trilinear value with 32-bit byte offsets from the volume's base (volumes hold
+// fewer than 2^30 voxels): one shared offset plus the +x/+y/+z steps, which are zero at
+// the upper faces (the reference's per-neighbour clamp).
+WAP_DEV float ldv(const float* __restrict__ base, unsigned off) {
+  return *(const float*)((const char*)base + off);
+}
+
+WAP_DEV float trilinear32(const float* __restrict__ vol, float p0, float p1, float p2, int W,
+                          int H, int D) {
+  const Corner c = clamp_pos(p0, p1, p2, W, H, D);
+  const unsigned o = 4u * ((unsigned)c.i0 + (unsigned)W * ((unsigned)c.i1 + (unsigned)H *
+                                                                               (unsigned)c.i2));
+  const unsigned ox = c.i0 < W - 1 ? 4u : 0u;
+  const unsigned oy = c.i1 < H - 1 ? 4u * (unsigned)W : 0u;
+  const unsigned oz = c.i2 < D - 1 ? 4u * (unsigned)W * (unsigned)H : 0u;
+  const float v000 = ldv(vol, o), v100 = ldv(vol, o + ox);
+  const float v010 = ldv(vol, o + oy), v110 = ldv(vol, o + oy + ox);
+  const float v001 = ldv(vol, o + oz), v101 = ldv(vol, o + oz + ox);
+  const float v011 = ldv(vol, o + oz + oy), v111 = ldv(vol, o + oz + oy + ox);
+  float g2 = c.f0, g1 = 1.0f - g2;
+  float a = v000 * g1 + v100 * g2;
+  float b = v010 * g1 + v110 * g2;
+  float cc = v001 * g1 + v101 * g2;
+  float d = v011 * g1 + v111 * g2;
+  g2 = c.f1;
+  g1 = 1.0f - g2;
+  a = a * g1 + b * g2;
+  cc = cc * g1 + d * g2;
+  g2 = c.f2;
+  g1 = 1.0f - g2;
+  return a * g1 + cc * g2;
+}
+
+// Forward-projection march: the same sample sequence as march() (which the scatter
+// uses), taking U samples per trip so their gathers are in flight together. Samples of
+// a trip past the segment end are gathered at their (clamped) positions but not counted
+// (adding +0 to a sum that started at +0 leaves it unchanged, so the accumulated bits
+// are those of the one-by-one march); t and the position advance by exactly the
+// counted steps, through the same sequence of additions.
+template <int U>
+WAP_DEV void march_project(const RaySeg& r, const float* __restrict__ V, int W, int H, int D,
+                           float sd, float& density, float& sw) {
+  const float tn = r.tn, tf = r.tf;
+  float t1 = tn;
+  float p1[3], step[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    p1[a] = r.z[a] + r.u[a] * t1;
+    step[a] = r.u[a] * sd;
+  }
+  sw += 1.0f;
+  density += trilinear32(V, p1[0], p1[1], p1[2], W, H, D) * sd;
+  const int bs[3] = {W, H, D};
+  float tMax[3], tDel[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    tDel[a] = 0.0f;
+    if (r.u[a] == 0.0f) {
+      tMax[a] = tf;
+    } else if (r.u[a] < 0.0f) {
+      const int x = ((int)ceilf(p1[a] - 0.5f) - 1) / bs[a];
+      tMax[a] = ((float)(x * bs[a]) + 0.5f - r.z[a]) / r.u[a];
+      tDel[a] = (float)(-bs[a]) / r.u[a];
+    } else {
+      const int x = (int)floorf(p1[a] - 0.5f) / bs[a];
+      tMax[a] = ((float)((x + 1) * bs[a]) + 0.5f - r.z[a]) / r.u[a];
+      tDel[a] = (float)bs[a] / r.u[a];
+    }
+  }
+  const float span = (float)(W + H + D);
+  const int cap = (int)(8.0f * (float)(W + H + D) / sd) + 16;
+  int visits = 1;
+  int segments = 0;
+  while (t1 < tf && segments++ < 16) {
+    float tb = mmin(mmin(tMax[0], tMax[1]), tMax[2]);
+    if (tb > tf) tb = tf;
+    if (tn - t1 > span) t1 = tn;
+    float t = ceilf((t1 - tn) / sd) * sd + tn;
+    float px = r.z[0] + r.u[0] * t, py = r.z[1] + r.u[1] * t, pz = r.z[2] + r.u[2] * t;
+    while (t < tb && visits < cap) {
+      float ts[U + 1], xs[U + 1], ys[U + 1], zs[U + 1], val[U];
+      ts[0] = t;
+      xs[0] = px;
+      ys[0] = py;
+      zs[0] = pz;
+#pragma unroll
+      for (int q = 0; q < U; ++q) {
+        ts[q + 1] = ts[q] + sd;
+        xs[q + 1] = xs[q] + step[0];
+        ys[q + 1] = ys[q] + step[1];
+        zs[q + 1] = zs[q] + step[2];
+      }
+#pragma unroll
+      for (int q = 0; q < U; ++q) val[q] = trilinear32(V, xs[q], ys[q], zs[q], W, H, D);
+      // sample 0 is valid (loop condition); later ones while t < tb and under the cap
+      int cnt = 1;
+      sw += 1.0f;
+      density += val[0] * sd;
+#pragma unroll
+      for (int q = 1; q < U; ++q) {
+        const bool ok = cnt == q && ts[q] < tb && visits + q < cap;
+        cnt += ok ? 1 : 0;
+        sw += ok ? 1.0f : 0.0f;
+        density += ok ? val[q] * sd : 0.0f;
+      }
+      visits += cnt;
+#pragma unroll
+      for (int q = 1; q <= U; ++q) {
+        if (cnt == q) {
+          t = ts[q];
+          px = xs[q];
+          py = ys[q];
+          pz = zs[q];
+        }
+      }
+    }
+    if (tMax[0] < tMax[1]) {
+      if (tMax[0] < tMax[2])
+        tMax[0] += tDel[0];
+      else
+        tMax[2] += tDel[2];
+    } else {
+      if (tMax[1] < tMax[2])
+        tMax[1] += tDel[1];
+      else
+        tMax[2] += tDel[2];
+    }
+    t1 = tb;
+  }
+}
+
 WAP_DEV void pixel_of(unsigned bid, unsigned tilesU, int& i, int& j) {
   const unsigned tu = bid % tilesU, tv = bid / tilesU;
   const unsigned lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
@@ -197,6 +328,7 @@ WAP_DEV void pixel_of(unsigned bid, unsigned tilesU, int& i, int& j) {
 // forward projection (+ optional SART correction epilogue)
 // grid: x = 16x16 pixel tiles of one view (XCD remapped), y = view in batch
 // ---------------------------------------------------------------------------------
+template <int U>
 __global__ __launch_bounds__(256) void k_project(
     const float* __restrict__ vol, size_t vol_stride, int W, int H, int D, float sd,
     const WapView* __restrict__ views, int nu, int nv, float* __restrict__ proj,
@@ -214,10 +346,7 @@ __global__ __launch_bounds__(256) void k_project(
   float density = 0.0f, sw = 0.0f, len = 0.0f;
   if (r.tn < r.tf) {
     len = r.tf - r.tn;
-    march(r, W, H, D, sd, [&](float x, float y, float z) {
-      sw += 1.0f;
-      density += trilinear(V, x, y, z, W, H, D) * sd;
-    });
+    march_project<U>(r, V, W, H, D, sd, density, sw);
   }
   const size_t k = (size_t)view * nu * nv + (size_t)j * nu + i;
   if (proj) proj[k] = density;
@@ -346,11 +475,23 @@ __global__ __launch_bounds__(256) void k_bp_voxel(float* __restrict__ vol, int W
 hipError_t wapk_project(const float* vol, size_t vol_stride, const int* dims, float sd,
                         const WapView* views_dev, int nviews, int nu, int nv, float* proj,
                         float* raylen, float* weight, const float* b, float* py, float* corr,
-                        double alpha, hipStream_t s) {
+                        double alpha, int unroll, hipStream_t s) {
+  // 32-bit byte offsets inside one volume
+  if ((size_t)dims[0] * dims[1] * dims[2] >= (1ull << 30)) return hipErrorInvalidValue;
   const unsigned tiles = ((nu + 15) / 16) * ((nv + 15) / 16);
-  hipLaunchKernelGGL(k_project, dim3(tiles, nviews), dim3(256), 0, s, vol, vol_stride, dims[0],
-                     dims[1], dims[2], sd, views_dev, nu, nv, proj, raylen, weight, b, py, corr,
-                     alpha);
+#define WAP_PJ(U)                                                                              \
+  hipLaunchKernelGGL(k_project<U>, dim3(tiles, nviews), dim3(256), 0, s, vol, vol_stride,     \
+                     dims[0], dims[1], dims[2], sd, views_dev, nu, nv, proj, raylen, weight, b, \
+                     py, corr, alpha)
+  switch (unroll) {
+    case 1: WAP_PJ(1); break;
+    case 2: WAP_PJ(2); break;
+    case 3: WAP_PJ(3); break;
+    case 6: WAP_PJ(6); break;
+    case 8: WAP_PJ(8); break;
+    default: WAP_PJ(4); break;
+  }
+#undef WAP_PJ
   return hipGetLastError();
 }
 

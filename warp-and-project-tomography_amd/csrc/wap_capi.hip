@@ -23,7 +23,7 @@
 
 // kernel launchers (projector.hip, advect.hip, flow.hip, imageops.hip)
 hipError_t wapk_project(const float*, size_t, const int*, float, const WapView*, int, int, int,
-                        float*, float*, float*, const float*, float*, float*, double,
+                        float*, float*, float*, const float*, float*, float*, double, int,
                         hipStream_t);
 hipError_t wapk_scatter(const float*, const int*, float, const WapView*, int, int, int, double*,
                         double*, hipStream_t);
@@ -92,6 +92,7 @@ struct WapContext {
   int flow_variant = -1;  // primal-dual iteration form; -1 = autotuned per grid size
   std::map<std::array<int, 3>, int> flow_tuned;  // autotuner choices by (W, H, D)
   int flow_last = 0;                              // form used by the last flow level
+  int project_unroll = 2;                         // ray samples per march trip
 };
 
 namespace {
@@ -508,7 +509,7 @@ int sart_prox_impl(WapContext* c, const WapLoopConfig* L, float* vol, const floa
       const int view = order[q];
       WAP_HIP(c, wapk_project(vol, 0, L->dims, L->sd, c->views + view, 1, L->nu, L->nv, nullptr,
                               nullptr, nullptr, sino + (size_t)view * nd,
-                              py + (size_t)(view - first) * nd, corr, L->alpha, st));
+                              py + (size_t)(view - first) * nd, corr, L->alpha, c->project_unroll, st));
       WAP_HIP(c, wapk_bp_voxel(vol, L->dims, corr, c->views + view, L->nu, L->nv, L->alpha,
                                L->gmax, lo, hi, 0, st));
     }
@@ -533,7 +534,7 @@ int warp_project_impl(WapContext* c, const WapLoopConfig* L, float* vols, const 
   const int fsel = rtime < 0.0f ? std::max(0, frame - 1) : std::min(nf - 1, frame);
   WAP_HIP(c, wapk_advect(L->dims, aabb, flows + (size_t)fsel * 3 * n, vol, 1, rtime, warped, st));
   WAP_HIP(c, wapk_project(warped, 0, L->dims, L->sd, c->views + view, 1, L->nu, L->nv, nullptr,
-                          nullptr, nullptr, b, nullptr, corr, L->alpha, st));
+                          nullptr, nullptr, b, nullptr, corr, L->alpha, c->project_unroll, st));
   WAP_HIP(c, wapk_fill(delta, n, 0.0f, st));
   const int lo[3] = {0, 0, 0};
   WAP_HIP(c, wapk_bp_voxel(delta, L->dims, corr, c->views + view, L->nu, L->nv, L->alpha, 0.0f,
@@ -571,6 +572,12 @@ int wap_set_tuning(WapContext* c, const char* key, int value) {
     c->flow_variant = value;
     return WAP_OK;
   }
+  if (strcmp(key, "project_unroll") == 0) {
+    if (value != 1 && value != 2 && value != 3 && value != 4 && value != 6 && value != 8)
+      return fail(c, WAP_E_ARG, "project_unroll must be 1, 2, 3, 4, 6 or 8");
+    c->project_unroll = value;
+    return WAP_OK;
+  }
   if (strcmp(key, "flow_retune") == 0) {
     c->flow_tuned.clear();
     return WAP_OK;
@@ -582,6 +589,10 @@ int wap_get_tuning(const WapContext* c, const char* key, int* value) {
   if (!c || !key || !value) return WAP_E_ARG;
   if (strcmp(key, "flow_variant") == 0) {
     *value = c->flow_variant;
+    return WAP_OK;
+  }
+  if (strcmp(key, "project_unroll") == 0) {
+    *value = c->project_unroll;
     return WAP_OK;
   }
   if (strcmp(key, "flow_variant_used") == 0) {
@@ -649,7 +660,7 @@ int wap_forward_project(WapContext* c, const float* vol, size_t vol_stride, cons
     return fail(c, WAP_E_ARG, "wap_forward_project: invalid arguments");
   WAP_TRY(check_views(c, view0, nviews));
   WAP_HIP(c, wapk_project(vol, vol_stride, dims, sd, c->views + view0, nviews, nu, nv, proj,
-                          raylen, weight, nullptr, nullptr, nullptr, 0.0, S(stream)));
+                          raylen, weight, nullptr, nullptr, nullptr, 0.0, c->project_unroll, S(stream)));
   return WAP_OK;
 }
 
@@ -660,7 +671,7 @@ int wap_project_correct(WapContext* c, const float* vol, const int* dims, float 
     return fail(c, WAP_E_ARG, "wap_project_correct: invalid arguments");
   WAP_TRY(check_views(c, view, 1));
   WAP_HIP(c, wapk_project(vol, 0, dims, sd, c->views + view, 1, nu, nv, proj, raylen, nullptr, b,
-                          py, corr, alpha, S(stream)));
+                          py, corr, alpha, c->project_unroll, S(stream)));
   return WAP_OK;
 }
 
