@@ -45,6 +45,10 @@ int wap_version(void);
  *   grid); 1..5 = one kernel with LDS plane tiles; 6..29 = one single-pass kernel whose
  *   threads march z over column tiles (tile and chunk shapes listed in flow.hip).
  *   "flow_retune" (set, any value): forget the autotuner's choices.
+ *   "flow_storage": precision form of the flow solver's state: 0 = float (default; the
+ *   reference's floats bit for bit), 1 = bfloat16, 2 = half storage of the duals y, the
+ *   gradient u and ut (v, residual sums and all arithmetic stay float/double); forms
+ *   1..5 of "flow_variant" need float storage.
  *   "project_unroll": ray samples per trip of the forward projector's march (1, 2, 3,
  *   4, 6, 8; default 2).
  *   get-only: "flow_variant_used" (form of the last flow level), "flow_variant_count".

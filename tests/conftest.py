@@ -36,3 +36,10 @@ def wctx():
     ctx = wap_amd.Context(0)
     yield ctx
     ctx.close()
+
+
+@pytest.fixture(scope="session")
+def W():
+    """The package's Python mirror of the C ABI."""
+    import wap_amd
+    return wap_amd

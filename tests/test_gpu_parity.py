@@ -28,12 +28,6 @@ def H(t):
     return t.detach().cpu().numpy()
 
 
-@pytest.fixture(scope="module")
-def W():
-    import wap_amd
-    return wap_amd
-
-
 @pytest.mark.parametrize("parallel", [False, True])
 @pytest.mark.parametrize("shape,nu,nv", [((18, 20, 24), 32, 24), ((33, 40, 37), 57, 49)])
 def test_forward_project(wctx, oracle, W, parallel, shape, nu, nv):
@@ -303,3 +297,29 @@ def test_tvtp_step_and_xbar(wctx, oracle, frame, shape, roi):
     ox = xb.copy()
     oracle.lib.orc_xbar_update_f32(d, np.array(roi, np.int32), o[0], o[1], ox)
     np.testing.assert_array_equal(H(xbar), ox)
+
+
+@pytest.mark.parametrize("unroll", [1, 2, 3, 4, 6, 8])
+def test_projector_unroll_identical(wctx, oracle, W, unroll):
+    """Every march unroll of the forward projector gives the oracle's floats (cone beam,
+    long rays through a 40^3 volume, views around the circle)."""
+    shape, nu, nv = (40, 40, 40), 61, 59
+    vol = helpers.blob_volume(shape, 5)
+    geo = helpers.cone_geometry(W, shape, nu, nv)
+    angles = [0.0, 31.0, 90.0, 147.5, 200.0, 271.0]
+    views = W.views(geo, angles)
+    wctx.upload_views(views)
+    prev = wctx.get_tuning("project_unroll")
+    wctx.set_tuning("project_unroll", unroll)
+    try:
+        n = len(angles)
+        proj = torch.zeros((n, nv, nu), device=DEV)
+        wt = torch.zeros_like(proj)
+        wctx.forward_project(T(vol), SD, 0, n, nu, nv, proj, None, wt)
+        P, Wt = H(proj), H(wt)
+        for k in range(n):
+            a, _, c = oracle.forward_project(vol, views[k], nu, nv, SD)
+            np.testing.assert_array_equal(Wt[k], c)
+            np.testing.assert_array_equal(P[k], a)
+    finally:
+        wctx.set_tuning("project_unroll", prev)

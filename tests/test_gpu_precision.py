@@ -1,0 +1,103 @@
+"""Precision form of the flow solver: the duals y, the gradient u and ut stored as half or
+bfloat16 (the primal v, the residual sums and every computation stay float / double).
+
+Bar, against the FP64 oracle, written per output (relative L2):
+  - multiscale flow field:   half <= 5e-4, bfloat16 <= 3e-3
+  - one outer WaP iteration: volumes <= 1e-4 for both; flows as the flow bar
+The float form (storage 0) stays bit-identical to the float oracle.
+"""
+import numpy as np
+import pytest
+
+import helpers
+from helpers import SD
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+DEV = "cuda:0"
+FLOW_BAR = {1: 3e-3, 2: 5e-4}
+
+
+def T(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).to(DEV)
+
+
+def H(t):
+    torch.cuda.synchronize()
+    return t.detach().cpu().numpy()
+
+
+def rel(a, b):
+    return float(np.linalg.norm((a - b).ravel()) / np.linalg.norm(b.ravel()))
+
+
+@pytest.fixture
+def storage_ctx(wctx):
+    prev = wctx.get_tuning("flow_storage")
+    yield wctx
+    wctx.set_tuning("flow_storage", prev)
+
+
+@pytest.mark.parametrize("storage", [0, 1, 2])
+def test_flow_multiscale_storage(storage_ctx, oracle, storage):
+    ctx = storage_ctx
+    shape = (32, 32, 32)
+    I1 = oracle.blur(helpers.blob_volume(shape, 3), 1.0)
+    I2 = oracle.advect(helpers.smooth_flow(shape, 3, 1.5), I1, 1.0)
+    args = (0.008, 0.05, 50, 3, 0.5, 0.8, 3, [2] * 8)
+    v64 = oracle.flow_multiscale(I1.astype(np.float64), I2.astype(np.float64), None, *args,
+                                 f64=True)
+    ctx.set_tuning("flow_storage", storage)
+    out = torch.zeros((3,) + shape, device=DEV)
+    ctx.flow_multiscale(T(I1), T(I2), None, *args, out)
+    g = H(out)
+    if storage == 0:
+        np.testing.assert_array_equal(g, oracle.flow_multiscale(I1, I2, None, *args))
+    else:
+        err = rel(g, v64)
+        print(f"storage {storage}: flow rel L2 vs FP64 oracle {err:.3e}")
+        assert err <= FLOW_BAR[storage]
+    assert np.abs(g).max() > 0.1
+
+
+@pytest.mark.parametrize("storage", [1, 2])
+def test_wap_iteration_storage(storage_ctx, oracle, W, storage):
+    import pyoracle
+    ctx = storage_ctx
+    shape, nu, nv, nstates, per = (24, 24, 24), 36, 36, 3, 4
+    nviews = nstates * per
+    geo = helpers.cone_geometry(W, shape, nu, nv)
+    views = W.views(geo, helpers.phantoms.golden_angles(nviews, 360.0))
+    base = helpers.blob_volume(shape, 11)
+    flow = helpers.smooth_flow(shape, 11, 1.0)
+    truth = np.stack([oracle.advect(flow, base, float(s) / nstates) for s in range(nstates)])
+    oviews = (pyoracle.WapView * nviews).from_buffer_copy(bytes(views))
+    sino = np.stack([oracle.forward_project(truth[i // per], oviews[i], nu, nv, SD)[0]
+                     for i in range(nviews)])
+    cfg = W.WapLoopConfig()
+    cfg.dims[:] = [shape[2], shape[1], shape[0]]
+    cfg.nstates, cfg.nviews, cfg.nu, cfg.nv = nstates, nviews, nu, nv
+    cfg.sd, cfg.alpha, cfg.gmax, cfg.nb_sart = SD, 0.2, float(sino.max()), 2
+    cfg.roi[:] = [2, 2, 2, 2, 2, 2, 3, 3]
+    cfg.cut[:] = [2] * 8
+    cfg.flow_tol, cfg.flow_lambda, cfg.flow_eta, cfg.flow_sigma = 0.008, 0.05, 0.5, 0.8
+    cfg.flow_iters, cfg.flow_warps, cfg.flow_scales = 50, 3, 2
+    cfg.do_flow, cfg.do_warp, cfg.timeunit = 1, 1, float(per)
+    ws = np.array([per * s for s in range(nstates)], np.int32)
+    we = ws + per - 1
+    ft = np.array([per * s + 1.5 for s in range(nstates)], np.float32)
+    mask = oracle.roi_mask(shape, list(cfg.roi))
+    ocfg = pyoracle.OrcLoop.from_buffer_copy(bytes(cfg))
+    rv, rf = oracle.wap_iteration(ocfg, oviews, sino.astype(np.float64), ws, we, ft,
+                                  np.zeros((nstates,) + shape), np.zeros((nstates, 3) + shape),
+                                  mask.astype(np.float64), f64=True)
+    ctx.set_tuning("flow_storage", storage)
+    ctx.upload_views(views)
+    vols = torch.zeros((nstates,) + shape, device=DEV)
+    flows = torch.zeros((nstates, 3) + shape, device=DEV)
+    ctx.iteration(cfg, vols, flows, T(sino), ws, we, ft, T(mask))
+    ev, ef = rel(H(vols), rv), rel(H(flows), rf)
+    print(f"storage {storage}: volumes {ev:.3e}, flows {ef:.3e} rel L2 vs FP64 oracle")
+    assert ev <= 1e-4
+    assert ef <= FLOW_BAR[storage]

@@ -51,12 +51,13 @@ WAP_DEV double block_sum(double v) {
   return s;
 }
 
-// u and ut over the ROI from the warped state (central differences, clamped borders).
+// u and ut over the ROI from the warped state (central differences, clamped borders),
+// stored as S.
+template <class S>
 __global__ __launch_bounds__(256) void k_flow_prep(FlowGrid G, const float* __restrict__ warped,
                                                    const float* __restrict__ I1,
                                                    const float* __restrict__ v,
-                                                   float* __restrict__ u,
-                                                   float* __restrict__ ut) {
+                                                   S* __restrict__ u, S* __restrict__ ut) {
   const int bx = G.W - G.r0 - G.r1, by = G.H - G.r2 - G.r3, bz = G.D - G.r4 - G.r5;
   const size_t nb = (size_t)bx * by * bz;
   const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -69,22 +70,33 @@ __global__ __launch_bounds__(256) void k_flow_prep(FlowGrid G, const float* __re
     return warped[vidx(iclamp(a, 0, G.W - 1), iclamp(b, 0, G.H - 1), iclamp(c, 0, G.D - 1),
                        G.W, G.H)];
   };
-  const float gx = (at(i + 1, j, k) - at(i - 1, j, k)) / 2.0f;
-  const float gy = (at(i, j + 1, k) - at(i, j - 1, k)) / 2.0f;
-  const float gz = (at(i, j, k + 1) - at(i, j, k - 1)) / 2.0f;
-  u[t] = gx;
-  u[n + t] = gy;
-  u[2 * n + t] = gz;
-  ut[t] = warped[t] - I1[t] - gx * v[t] - gy * v[n + t] - gz * v[2 * n + t];
+  // the stored (possibly rounded) gradient is the one the iteration uses, so ut is
+  // formed from it
+  const float gx = to_f(from_f<S>((at(i + 1, j, k) - at(i - 1, j, k)) / 2.0f));
+  const float gy = to_f(from_f<S>((at(i, j + 1, k) - at(i, j - 1, k)) / 2.0f));
+  const float gz = to_f(from_f<S>((at(i, j, k + 1) - at(i, j, k - 1)) / 2.0f));
+  u[t] = from_f<S>(gx);
+  u[n + t] = from_f<S>(gy);
+  u[2 * n + t] = from_f<S>(gz);
+  ut[t] = from_f<S>(warped[t] - I1[t] - gx * v[t] - gy * v[n + t] - gz * v[2 * n + t]);
 }
 
 // u.v of the outgoing warp, needed as the previous K*x on the first iteration of the
 // next warp (u changes between warps, v does not).
-__global__ void k_flow_kx4(size_t n, const float* __restrict__ u, const float* __restrict__ v,
+template <class S>
+__global__ void k_flow_kx4(size_t n, const S* __restrict__ u, const float* __restrict__ v,
                            float* __restrict__ kx4) {
   const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
-  kx4[t] = u[t] * v[t] + u[n + t] * v[n + t] + u[2 * n + t] * v[2 * n + t];
+  kx4[t] = to_f(u[t]) * v[t] + to_f(u[n + t]) * v[n + t] + to_f(u[2 * n + t]) * v[2 * n + t];
+}
+
+// Backward difference of a stored field (see dminus).
+template <class S>
+WAP_DEV float dminus_s(const S* __restrict__ a, size_t t, int c, int n, size_t stride) {
+  if (c > 0 && c < n - 1) return to_f(a[t]) - to_f(a[t - stride]);
+  if (c == 0) return to_f(a[t]);
+  return -to_f(a[t - stride]);
 }
 
 WAP_DEV bool in_roi(const FlowGrid& G, int i, int j, int k) {
@@ -230,9 +242,10 @@ __global__ __launch_bounds__(TX * TY) void k_flow_pd(
 // voxel): more HBM traffic than the fused form but no recomputation; kept as a tuning
 // variant.
 // primal step over the ROI: vdst = vsrc - tau*K^T y  (vdst holds vsrc outside the ROI)
+template <class S>
 __global__ __launch_bounds__(256) void k_flow_primal(
     FlowGrid G, const float* __restrict__ vsrc, float* __restrict__ vdst,
-    const float* __restrict__ y, const float* __restrict__ u, float* __restrict__ kty,
+    const S* __restrict__ y, const S* __restrict__ u, float* __restrict__ kty,
     int store_kty, int resid, double* __restrict__ ppart, const int* __restrict__ ctl) {
   if (ctl[0] == 0) return;
   const int bx = G.W - G.r0 - G.r1, by = G.H - G.r2 - G.r3, bz = G.D - G.r4 - G.r5;
@@ -245,14 +258,14 @@ __global__ __launch_bounds__(256) void k_flow_primal(
     const size_t n = (size_t)G.W * G.H * G.D;
     const size_t t = vidx(i, j, k, G.W, G.H);
     const size_t sy = (size_t)G.W, sz = (size_t)G.W * G.H;
-    const float y4 = y[9 * n + t];
+    const float y4 = to_f(y[9 * n + t]);
     float term = 0.0f;
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-      const float uc = u[c * n + t];
-      const float K = -1.0f * dminus(y + (3 * c + 0) * n, t, i, G.W, 1) -
-                      1.0f * dminus(y + (3 * c + 1) * n, t, j, G.H, sy) -
-                      1.0f * dminus(y + (3 * c + 2) * n, t, k, G.D, sz) + uc * y4;
+      const float uc = to_f(u[c * n + t]);
+      const float K = -1.0f * dminus_s(y + (3 * c + 0) * n, t, i, G.W, 1) -
+                      1.0f * dminus_s(y + (3 * c + 1) * n, t, j, G.H, sy) -
+                      1.0f * dminus_s(y + (3 * c + 2) * n, t, k, G.D, sz) + uc * y4;
       const float tau = 1.0f / (4.0f / 1.0f + my_abs(uc));
       const float vo = vsrc[c * n + t];
       const float vn = vo - tau * K;
@@ -270,11 +283,12 @@ __global__ __launch_bounds__(256) void k_flow_primal(
 
 // dual step over every voxel; mode 0: previous K*x = 0 (first iteration of the solve),
 // mode 1: previous u.v from kx4prev (first iteration of a later warp), mode 2: normal
+template <class S>
 __global__ __launch_bounds__(256) void k_flow_dual(
     FlowGrid G, const float* __restrict__ vold, const float* __restrict__ vnew,
-    const float* __restrict__ y, const float* __restrict__ u, const float* __restrict__ ut,
+    const S* __restrict__ y, const S* __restrict__ u, const S* __restrict__ ut,
     const float* __restrict__ kx4prev, int mode, int resid, float sigma1, float lambda,
-    double* __restrict__ dpart, int* __restrict__ ctl, int iter, float* __restrict__ yout) {
+    double* __restrict__ dpart, int* __restrict__ ctl, int iter, S* __restrict__ yout) {
   if (ctl[0] == 0) return;
   const size_t n = (size_t)G.W * G.H * G.D;
   const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -284,7 +298,7 @@ __global__ __launch_bounds__(256) void k_flow_dual(
     const int i = (int)(t % G.W), j = (int)((t / G.W) % G.H), k = (int)(t / ((size_t)G.W * G.H));
     const size_t off[3] = {1, (size_t)G.W, (size_t)G.W * G.H};
     const bool has[3] = {i < G.W - 1, j < G.H - 1, k < G.D - 1};
-    const float ux = u[t], uy = u[n + t], uz = u[2 * n + t];
+    const float ux = to_f(u[t]), uy = to_f(u[n + t]), uz = to_f(u[2 * n + t]);
     float vn[3], vo[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
@@ -301,10 +315,10 @@ __global__ __launch_bounds__(256) void k_flow_dual(
         const float ko =
             mode == 0 ? 0.0f : (has[a] ? 1.0f * (vold[c * n + t + off[a]] - vo[c]) : 0.0f);
         const size_t p = (size_t)(3 * c + a) * n + t;
-        const float yo = y[p];
+        const float yo = to_f(y[p]);
         const float yt = yo + sigma1 * (kx + kx - ko);
-        const float yn = yt * lambda / shrink_den;
-        yout[p] = yn;
+        const float yn = to_f(from_f<S>(yt * lambda / shrink_den));
+        yout[p] = from_f<S>(yn);
         if (resid) term += fabsf((yo - yn) / sigma1 - ko + kx);
       }
     }
@@ -312,11 +326,11 @@ __global__ __launch_bounds__(256) void k_flow_dual(
     const float k4o =
         mode == 0 ? 0.0f : (mode == 1 ? kx4prev[t] : ux * vo[0] + uy * vo[1] + uz * vo[2]);
     const float s2 = 1.0f / (fabsf(ux) + fabsf(uy) + fabsf(uz));
-    const float y4o = y[9 * n + t];
-    const float y4t = y4o + s2 * (k4 + k4 - k4o + ut[t]);
+    const float y4o = to_f(y[9 * n + t]);
+    const float y4t = y4o + s2 * (k4 + k4 - k4o + to_f(ut[t]));
     const float m1 = (y4t < 1.0f) ? y4t : 1.0f;
-    const float y4n = (-1.0f < m1) ? m1 : -1.0f;
-    yout[9 * n + t] = y4n;
+    const float y4n = to_f(from_f<S>((-1.0f < m1) ? m1 : -1.0f));
+    yout[9 * n + t] = from_f<S>(y4n);
     if (resid) {
       term += fabsf((y4o - y4n) / s2 - k4o + k4);
       dterm = (double)term;
@@ -363,9 +377,9 @@ WAP_DEV void stb(float* __restrict__ base, unsigned off, float v) {
 // Every load is unconditional, at an in-bounds (clamped) address, and the boundary and
 // ROI rules pick among the loaded values with selects: a load inside a divergent branch
 // would make the wave wait for it before leaving the branch, serialising the step.
-template <bool FULL>
+template <bool FULL, class S>
 WAP_DEV void pd_point(const FlowGrid& G, const float* __restrict__ vo,
-                      const float* __restrict__ yo, const float* __restrict__ u, size_t n,
+                      const S* __restrict__ yo, const S* __restrict__ u, size_t n,
                       int i, int j, int k, unsigned tb, bool roi, float* c3, PdPoint<FULL>& p) {
   const unsigned syb = 4u * (unsigned)G.W;
   const unsigned xm = i > 0 ? tb - 4u : tb, ym = j > 0 ? tb - syb : tb;
@@ -373,14 +387,14 @@ WAP_DEV void pd_point(const FlowGrid& G, const float* __restrict__ vo,
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
     p.vo[c] = ldb(vo + c * n, tb);
-    y1[c] = ldb(yo + (3 * c) * n, tb);
-    y2[c] = ldb(yo + (3 * c + 1) * n, tb);
-    y3[c] = ldb(yo + (3 * c + 2) * n, tb);
-    y1m[c] = ldb(yo + (3 * c) * n, xm);
-    y2m[c] = ldb(yo + (3 * c + 1) * n, ym);
-    uu[c] = ldb(u + c * n, tb);
+    y1[c] = lds(yo + (3 * c) * n, tb);
+    y2[c] = lds(yo + (3 * c + 1) * n, tb);
+    y3[c] = lds(yo + (3 * c + 2) * n, tb);
+    y1m[c] = lds(yo + (3 * c) * n, xm);
+    y2m[c] = lds(yo + (3 * c + 1) * n, ym);
+    uu[c] = lds(u + c * n, tb);
   }
-  const float y4 = ldb(yo + 9 * n, tb);
+  const float y4 = lds(yo + 9 * n, tb);
   if constexpr (FULL) {
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
@@ -418,11 +432,11 @@ WAP_DEV void pd_point(const FlowGrid& G, const float* __restrict__ vo,
 // plane step is issued before the step's stores. Threads past the grid edge run the
 // march on clamped coordinates (so every barrier is reached) and store nothing.
 // KTY: this iteration keeps K^T y (store_kty) or sums the residual terms (resid).
-template <bool KTY, bool XLDS, bool YLDS>
+template <bool KTY, bool XLDS, bool YLDS, class S>
 __global__ __launch_bounds__(1024) void k_flow_zm(
     FlowGrid G, const float* __restrict__ vo, float* __restrict__ vn,
-    const float* __restrict__ yo, float* __restrict__ yn, const float* __restrict__ u,
-    const float* __restrict__ ut, const float* __restrict__ kx4prev, float* __restrict__ kty,
+    const S* __restrict__ yo, S* __restrict__ yn, const S* __restrict__ u,
+    const S* __restrict__ ut, const float* __restrict__ kx4prev, float* __restrict__ kty,
     int zc, int mode, int resid, int store_kty, float sigma1, float lambda,
     double* __restrict__ ppart, double* __restrict__ dpart, int* __restrict__ ctl, int iter) {
   extern __shared__ float xsh[];  // XLDS: 2 x 6 x (threads) floats
@@ -461,19 +475,19 @@ __global__ __launch_bounds__(1024) void k_flow_zm(
   if (z0 > 0) {
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-      const float* y3 = yo + (3 * c + 2) * n;
-      c3o[c] = ldb(y3, t - szb);
-      if (!XLDS) c3x[c] = ldb(y3, t + dxb - szb);
-      if (!YLDS || ylast) c3y[c] = ldb(y3, t + dyb - szb);
+      const S* y3 = yo + (3 * c + 2) * n;
+      c3o[c] = lds(y3, t - szb);
+      if (!XLDS) c3x[c] = lds(y3, t + dxb - szb);
+      if (!YLDS || ylast) c3y[c] = lds(y3, t + dyb - szb);
     }
   }
   auto rz = [&](int k) { return k >= G.r4 && k < D - G.r5; };
   PdPoint<true> o;
   PdPoint<false> nx, ny;
-  pd_point<true>(G, vo, yo, u, n, i, j, z0, t, roi_o && rz(z0), c3o, o);
-  if (!XLDS) pd_point<false>(G, vo, yo, u, n, ix, j, z0, t + dxb, roi_x && rz(z0), c3x, nx);
+  pd_point<true, S>(G, vo, yo, u, n, i, j, z0, t, roi_o && rz(z0), c3o, o);
+  if (!XLDS) pd_point<false, S>(G, vo, yo, u, n, ix, j, z0, t + dxb, roi_x && rz(z0), c3x, nx);
   if (!YLDS || ylast)
-    pd_point<false>(G, vo, yo, u, n, i, jy, z0, t + dyb, roi_y && rz(z0), c3y, ny);
+    pd_point<false, S>(G, vo, yo, u, n, i, jy, z0, t + dyb, roi_y && rz(z0), c3y, ny);
   for (int k = z0; k < z1; ++k) {
     const bool hz = k < D - 1;
     const bool more = k + 1 < z1;
@@ -484,12 +498,12 @@ __global__ __launch_bounds__(1024) void k_flow_zm(
     const unsigned tn = hz ? t + szb : t;
     PdPoint<true> on;
     PdPoint<false> nx1, ny1;
-    pd_point<true>(G, vo, yo, u, n, i, j, kn, tn, roi_o && rz(kn), c3o, on);
+    pd_point<true, S>(G, vo, yo, u, n, i, j, kn, tn, roi_o && rz(kn), c3o, on);
     if (!XLDS)
-      pd_point<false>(G, vo, yo, u, n, ix, j, kn, tn + dxb, roi_x && rz(kn), c3x, nx1);
+      pd_point<false, S>(G, vo, yo, u, n, ix, j, kn, tn + dxb, roi_x && rz(kn), c3x, nx1);
     if (!YLDS || ylast)
-      pd_point<false>(G, vo, yo, u, n, i, jy, kn, tn + dyb, roi_y && rz(kn), c3y, ny1);
-    const float utt = ldb(ut, t);
+      pd_point<false, S>(G, vo, yo, u, n, i, jy, kn, tn + dyb, roi_y && rz(kn), c3y, ny1);
+    const float utt = lds(ut, t);
     const float k4p = ldb(kx4prev, t);
     const float ktyo[3] = {KTY && resid ? ldb(kty, t) : 0.0f,
                            KTY && resid ? ldb(kty + n, t) : 0.0f,
@@ -530,8 +544,8 @@ __global__ __launch_bounds__(1024) void k_flow_zm(
         const float ko = mode == 0 ? 0.0f : (has[a] ? 1.0f * (nbo[a] - o.vo[c]) : 0.0f);
         const float yv = o.y[3 * c + a];
         const float yt = yv + sigma1 * (kx + kx - ko);
-        const float ynv = yt * lambda / shrink_den;
-        if (act) stb(yn + (3 * c + a) * n, t, ynv);
+        const float ynv = to_f(from_f<S>(yt * lambda / shrink_den));
+        if (act) sts(yn + (3 * c + a) * n, t, ynv);
         if (KTY) dt += fabsf((yv - ynv) / sigma1 - ko + kx);
       }
       if (act) stb(vn + c * n, t, o.vn[c]);
@@ -544,8 +558,8 @@ __global__ __launch_bounds__(1024) void k_flow_zm(
     const float y4o = o.y[9];
     const float y4t = y4o + s2 * (k4 + k4 - k4o + utt);
     const float m1 = (y4t < 1.0f) ? y4t : 1.0f;
-    const float y4n = (-1.0f < m1) ? m1 : -1.0f;
-    if (act) stb(yn + 9 * n, t, y4n);
+    const float y4n = to_f(from_f<S>((-1.0f < m1) ? m1 : -1.0f));
+    if (act) sts(yn + 9 * n, t, y4n);
     if (KTY && act) {
       if (resid) {
         dt += fabsf((y4o - y4n) / s2 - k4o + k4);
@@ -602,9 +616,10 @@ __global__ __launch_bounds__(256) void k_flow_resid(const double* __restrict__ p
 // After a warp: the host alternated the v and y buffers for every queued iteration;
 // if the device stopped early with the other parity, move the live state into the
 // buffers the host treats as current.
+template <class S>
 __global__ void k_flow_settle(size_t n, int host_last, float* __restrict__ v_host,
-                              const float* __restrict__ v_other, float* __restrict__ y_host,
-                              const float* __restrict__ y_other, const int* __restrict__ ctl) {
+                              const float* __restrict__ v_other, S* __restrict__ y_host,
+                              const S* __restrict__ y_other, const int* __restrict__ ctl) {
   const int dev_last = ctl[1];
   const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (!((dev_last ^ host_last) & 1) || t >= n) return;
@@ -612,6 +627,27 @@ __global__ void k_flow_settle(size_t n, int host_last, float* __restrict__ v_hos
   for (int c = 0; c < 3; ++c) v_host[c * n + t] = v_other[c * n + t];
 #pragma unroll
   for (int c = 0; c < 10; ++c) y_host[c * n + t] = y_other[c * n + t];
+}
+
+// conversions between the caller's float duals and the stored form
+template <class S>
+__global__ void k_to_store(const float* __restrict__ src, S* __restrict__ dst, size_t n) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) dst[t] = from_f<S>(src[t]);
+}
+
+template <class S>
+__global__ void k_box_from_store(float* __restrict__ dst, const S* __restrict__ src, int W,
+                                 int H, int D, int nc, int x0, int y0, int z0, int bx, int by,
+                                 int bz) {
+  const size_t nb = (size_t)bx * by * bz;
+  const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nb * nc) return;
+  const int c = (int)(q / nb);
+  const size_t r = q % nb;
+  const int i = x0 + (int)(r % bx), j = y0 + (int)((r / bx) % by), k = z0 + (int)(r / ((size_t)bx * by));
+  const size_t t = (size_t)c * W * H * D + vidx(i, j, k, W, H);
+  dst[t] = to_f(src[t]);
 }
 
 __global__ void k_zero_tail(double* __restrict__ p, size_t n) {
@@ -719,40 +755,83 @@ size_t wapk_flow_blocks(int variant, const int* dims) {
          ((dims[2] + p.zc - 1) / p.zc);
 }
 
-hipError_t wapk_flow_prep(const int* dims, const int* roi, const float* warped, const float* I1,
-                          const float* v, float* u, float* ut, hipStream_t s) {
+// Storage codes of the flow state (duals, gradient, ut): 0 float, 1 bfloat16, 2 half
+// (the WapDType values). Each launcher instantiates its kernel for the code given.
+#define WAP_BY_STORAGE(code, CALL) \
+  do {                             \
+    if ((code) == 1) {             \
+      typedef Bf16 S;              \
+      CALL;                        \
+    } else if ((code) == 2) {      \
+      typedef _Float16 S;          \
+      CALL;                        \
+    } else {                       \
+      typedef float S;             \
+      CALL;                        \
+    }                              \
+  } while (0)
+
+size_t wapk_flow_elem(int storage) { return storage == 1 || storage == 2 ? 2 : 4; }
+
+hipError_t wapk_flow_prep(int storage, const int* dims, const int* roi, const float* warped,
+                          const float* I1, const float* v, void* u, void* ut, hipStream_t s) {
   const size_t nb = roi_count(dims, roi);
   if (nb == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_flow_prep, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, s,
-                     mkgrid(dims, roi), warped, I1, v, u, ut);
+  WAP_BY_STORAGE(storage, hipLaunchKernelGGL(k_flow_prep<S>, dim3((unsigned)((nb + 255) / 256)),
+                                             dim3(256), 0, s, mkgrid(dims, roi), warped, I1, v,
+                                             (S*)u, (S*)ut));
   return hipGetLastError();
 }
 
-hipError_t wapk_flow_kx4(const int* dims, const float* u, const float* v, float* kx4,
+hipError_t wapk_flow_kx4(int storage, const int* dims, const void* u, const float* v, float* kx4,
                          hipStream_t s) {
   const size_t n = (size_t)dims[0] * dims[1] * dims[2];
-  hipLaunchKernelGGL(k_flow_kx4, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, u, v,
-                     kx4);
+  WAP_BY_STORAGE(storage, hipLaunchKernelGGL(k_flow_kx4<S>, dim3((unsigned)((n + 255) / 256)),
+                                             dim3(256), 0, s, n, (const S*)u, v, kx4));
+  return hipGetLastError();
+}
+
+hipError_t wapk_to_store(int storage, const float* src, void* dst, size_t n, hipStream_t s) {
+  WAP_BY_STORAGE(storage, hipLaunchKernelGGL(k_to_store<S>, dim3((unsigned)((n + 255) / 256)),
+                                             dim3(256), 0, s, src, (S*)dst, n));
+  return hipGetLastError();
+}
+
+hipError_t wapk_box_from_store(int storage, float* dst, const void* src, const int* dims, int nc,
+                               const int* lo, const int* hi, hipStream_t s) {
+  const int bx = hi[0] - lo[0], by = hi[1] - lo[1], bz = hi[2] - lo[2];
+  if (bx <= 0 || by <= 0 || bz <= 0) return hipSuccess;
+  const size_t tot = (size_t)bx * by * bz * nc;
+  WAP_BY_STORAGE(storage,
+                 hipLaunchKernelGGL(k_box_from_store<S>, dim3((unsigned)((tot + 255) / 256)),
+                                    dim3(256), 0, s, dst, (const S*)src, dims[0], dims[1],
+                                    dims[2], nc, lo[0], lo[1], lo[2], bx, by, bz));
   return hipGetLastError();
 }
 
 #define WAP_PD_LAUNCH(TX, TY, ZC)                                                        \
-  hipLaunchKernelGGL((k_flow_pd<TX, TY, ZC>), dim3(nb), dim3(256), 0, s, G, vold, vnew, yold, \
-                     ynew, u, ut, kx4prev, kty, mode, resid, store_kty, sigma1, lambda, ppart,  \
-                     dpart, ctl, iter)
+  hipLaunchKernelGGL((k_flow_pd<TX, TY, ZC>), dim3(nb), dim3(256), 0, s, G, vold, vnew,       \
+                     (const float*)yold, (float*)ynew, (const float*)u, (const float*)ut,      \
+                     kx4prev, kty, mode, resid, store_kty, sigma1, lambda, ppart, dpart, ctl,   \
+                     iter)
 
-hipError_t wapk_flow_pd(int variant, const int* dims, const int* roi, const float* vold,
-                        float* vnew, const float* yold, float* ynew, const float* u,
-                        const float* ut, const float* kx4prev, float* kty, int mode, int resid,
-                        int store_kty, float sigma1, float lambda, double* ppart, double* dpart,
-                        int* ctl, int iter, hipStream_t s) {
+// One primal-dual iteration in the form `variant`; y, u and ut are stored as `storage`
+// (the LDS plane-tile forms 1..5 take float storage only).
+hipError_t wapk_flow_pd(int variant, int storage, const int* dims, const int* roi,
+                        const float* vold, float* vnew, const void* yold, void* ynew,
+                        const void* u, const void* ut, const float* kx4prev, float* kty,
+                        int mode, int resid, int store_kty, float sigma1, float lambda,
+                        double* ppart, double* dpart, int* ctl, int iter, hipStream_t s) {
   const FlowGrid G = mkgrid(dims, roi);
   const unsigned nb = (unsigned)wapk_flow_blocks(variant, dims);
   if (variant == 0) {
     const size_t nr = roi_count(dims, roi);
     if (nr)
-      hipLaunchKernelGGL(k_flow_primal, dim3((unsigned)((nr + 255) / 256)), dim3(256), 0, s, G,
-                         vold, vnew, yold, u, kty, store_kty, resid, ppart, ctl);
+      WAP_BY_STORAGE(storage, hipLaunchKernelGGL(k_flow_primal<S>,
+                                                 dim3((unsigned)((nr + 255) / 256)), dim3(256),
+                                                 0, s, G, vold, vnew, (const S*)yold,
+                                                 (const S*)u, kty, store_kty, resid, ppart,
+                                                 ctl));
     if (resid) {
       // the primal sums sit in the first ceil(nr/256) slots; clear the rest
       const size_t used = (nr + 255) / 256;
@@ -760,8 +839,10 @@ hipError_t wapk_flow_pd(int variant, const int* dims, const int* roi, const floa
         hipLaunchKernelGGL(k_zero_tail, dim3((unsigned)((nb - used + 255) / 256)), dim3(256), 0,
                            s, ppart + used, nb - used);
     }
-    hipLaunchKernelGGL(k_flow_dual, dim3(nb), dim3(256), 0, s, G, vold, vnew, yold, u, ut,
-                       kx4prev, mode, resid, sigma1, lambda, dpart, ctl, iter, ynew);
+    WAP_BY_STORAGE(storage, hipLaunchKernelGGL(k_flow_dual<S>, dim3(nb), dim3(256), 0, s, G,
+                                               vold, vnew, (const S*)yold, (const S*)u,
+                                               (const S*)ut, kx4prev, mode, resid, sigma1,
+                                               lambda, dpart, ctl, iter, (S*)ynew));
     return hipGetLastError();
   }
   if (variant >= 6) {
@@ -771,11 +852,13 @@ hipError_t wapk_flow_pd(int variant, const int* dims, const int* roi, const floa
     const dim3 b(z.bx, z.by);
     const int zc = z.zc;
     const int r = resid, st = store_kty;
-#define WAP_ZM(KT, XL, YL, R, ST)                                                          \
-  hipLaunchKernelGGL((k_flow_zm<KT, XL, YL>), dim3(nb), b, XL ? lds : 0, s, G, vold, vnew, yold, \
-                     ynew, u, ut, kx4prev, kty, zc, mode, R, ST, sigma1, lambda, ppart, dpart,   \
-                     ctl, iter)
-    const size_t lds = sizeof(float) * 2 * 6 * z.bx * z.by;
+    const size_t ldsz = sizeof(float) * 2 * 6 * z.bx * z.by;
+#define WAP_ZM(KT, XL, YL, R, ST)                                                              \
+  WAP_BY_STORAGE(storage, hipLaunchKernelGGL((k_flow_zm<KT, XL, YL, S>), dim3(nb), b,          \
+                                             XL ? ldsz : 0, s, G, vold, vnew, (const S*)yold,  \
+                                             (S*)ynew, (const S*)u, (const S*)ut, kx4prev, kty, \
+                                             zc, mode, R, ST, sigma1, lambda, ppart, dpart,     \
+                                             ctl, iter))
     if (r || st) {
       if (z.ylds) WAP_ZM(true, true, true, r, st);
       else if (z.xlds) WAP_ZM(true, true, false, r, st);
@@ -788,6 +871,7 @@ hipError_t wapk_flow_pd(int variant, const int* dims, const int* roi, const floa
 #undef WAP_ZM
     return hipGetLastError();
   }
+  if (storage != 0) return hipErrorInvalidValue;
   const PdShape p = shape_of(variant, dims);
   if (p.tx == 16) {
     if (p.zc == 2) WAP_PD_LAUNCH(16, 16, 2);
@@ -812,10 +896,12 @@ hipError_t wapk_flow_resid(const double* ppart, int np, const double* dpart, int
   return hipGetLastError();
 }
 
-hipError_t wapk_flow_settle(size_t n, int host_last, float* v_host, const float* v_other,
-                            float* y_host, const float* y_other, const int* ctl, hipStream_t s) {
-  hipLaunchKernelGGL(k_flow_settle, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n,
-                     host_last, v_host, v_other, y_host, y_other, ctl);
+hipError_t wapk_flow_settle(int storage, size_t n, int host_last, float* v_host,
+                            const float* v_other, void* y_host, const void* y_other,
+                            const int* ctl, hipStream_t s) {
+  WAP_BY_STORAGE(storage, hipLaunchKernelGGL(k_flow_settle<S>, dim3((unsigned)((n + 255) / 256)),
+                                             dim3(256), 0, s, n, host_last, v_host, v_other,
+                                             (S*)y_host, (const S*)y_other, ctl));
   return hipGetLastError();
 }
 

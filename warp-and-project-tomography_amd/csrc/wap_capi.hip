@@ -35,15 +35,19 @@ hipError_t wapk_advect(const int*, const float*, const float*, const float*, int
                        hipStream_t);
 int wapk_flow_variants();
 size_t wapk_flow_blocks(int, const int*);
-hipError_t wapk_flow_prep(const int*, const int*, const float*, const float*, const float*,
-                          float*, float*, hipStream_t);
-hipError_t wapk_flow_kx4(const int*, const float*, const float*, float*, hipStream_t);
-hipError_t wapk_flow_pd(int, const int*, const int*, const float*, float*, const float*, float*,
-                        const float*, const float*, const float*, float*, int, int, int, float,
-                        float, double*, double*, int*, int, hipStream_t);
+size_t wapk_flow_elem(int);
+hipError_t wapk_flow_prep(int, const int*, const int*, const float*, const float*, const float*,
+                          void*, void*, hipStream_t);
+hipError_t wapk_flow_kx4(int, const int*, const void*, const float*, float*, hipStream_t);
+hipError_t wapk_to_store(int, const float*, void*, size_t, hipStream_t);
+hipError_t wapk_box_from_store(int, float*, const void*, const int*, int, const int*, const int*,
+                               hipStream_t);
+hipError_t wapk_flow_pd(int, int, const int*, const int*, const float*, float*, const void*,
+                        void*, const void*, const void*, const float*, float*, int, int, int,
+                        float, float, double*, double*, int*, int, hipStream_t);
 hipError_t wapk_flow_resid(const double*, int, const double*, int, int, float, int*, float*,
                            hipStream_t);
-hipError_t wapk_flow_settle(size_t, int, float*, const float*, float*, const float*,
+hipError_t wapk_flow_settle(int, size_t, int, float*, const float*, void*, const void*,
                             const int*, hipStream_t);
 hipError_t wapk_flow_arm(int*, hipStream_t);
 hipError_t wapk_deriche(const float*, float*, const int*, int, const float*, hipStream_t);
@@ -90,7 +94,8 @@ struct WapContext {
   std::map<std::string, DevBuf> bufs;
   std::map<std::pair<int, int>, ResizeTable> tables;
   int flow_variant = -1;  // primal-dual iteration form; -1 = autotuned per grid size
-  std::map<std::array<int, 3>, int> flow_tuned;  // autotuner choices by (W, H, D)
+  std::map<std::array<int, 4>, int> flow_tuned;  // autotuner choices by (W, H, D, storage)
+  int flow_storage = 0;  // storage of the flow duals, gradient and ut (WapDType)
   int flow_last = 0;                              // form used by the last flow level
   int project_unroll = 2;                         // ray samples per march trip
 };
@@ -306,9 +311,9 @@ int blur_impl(WapContext* c, const float* src, const int* dims, int nc, float si
 // keep the fastest for that size. The timed launches read (vcur, ycur) and write
 // (vnext, ynext), which the level then overwrites, so the state is untouched.
 int flow_tune(WapContext* c, const int* dims, const int* roi, const float* vcur, float* vnext,
-              const float* ycur, float* ynext, const float* u, const float* ut,
-              const float* kx4, float* kty, int* ctl, hipStream_t st, int* chosen) {
-  const std::array<int, 3> key = {dims[0], dims[1], dims[2]};
+              const void* ycur, void* ynext, const void* u, const void* ut, const float* kx4,
+              float* kty, int* ctl, hipStream_t st, int* chosen) {
+  const std::array<int, 4> key = {dims[0], dims[1], dims[2], c->flow_storage};
   auto it = c->flow_tuned.find(key);
   if (it != c->flow_tuned.end()) {
     *chosen = it->second;
@@ -328,8 +333,8 @@ int flow_tune(WapContext* c, const int* dims, const int* roi, const float* vcur,
     for (int r = 0; r < 4 && e == hipSuccess; ++r) {
       if (r == 1) e = hipEventRecord(e0, st);
       if (e == hipSuccess)
-        e = wapk_flow_pd(v, dims, roi, vcur, vnext, ycur, ynext, u, ut, kx4, kty, 2, 0, 0,
-                         1.0f / 3.0f, 0.05f, nullptr, nullptr, ctl, 0, st);
+        e = wapk_flow_pd(v, c->flow_storage, dims, roi, vcur, vnext, ycur, ynext, u, ut, kx4,
+                         kty, 2, 0, 0, 1.0f / 3.0f, 0.05f, nullptr, nullptr, ctl, 0, st);
     }
     if (e == hipSuccess) e = hipEventRecord(e1, st);
     if (e == hipSuccess) e = hipEventSynchronize(e1);
@@ -358,15 +363,19 @@ int flow_level(WapContext* c, const int* dims, const float* I1, const float* I2,
   const int lo[3] = {roi[0], roi[2], roi[4]};
   const int hi[3] = {dims[0] - roi[1], dims[1] - roi[3], dims[2] - roi[5]};
   const bool has_roi = hi[0] > lo[0] && hi[1] > lo[1] && hi[2] > lo[2];
+  const int sto = c->flow_storage;
+  const size_t es = wapk_flow_elem(sto);
+  // y (10 planes), u (3) and ut (1) in the storage type; allocated in whole floats
+  auto fl4 = [&](size_t elems) { return (elems * es + 3) / 4; };
   float *vB, *yA, *yB, *u, *ut, *warped, *kty, *kx4;
   double *pp, *dp;
   int* ctl;
   float* err;
   WAP_TRY(scratch(c, "flow_vB", 3 * n, &vB));
-  WAP_TRY(scratch(c, "flow_yA", 10 * n, &yA));
-  WAP_TRY(scratch(c, "flow_yB", 10 * n, &yB));
-  WAP_TRY(scratch(c, "flow_u", 3 * n, &u));
-  WAP_TRY(scratch(c, "flow_ut", n, &ut));
+  WAP_TRY(scratch(c, "flow_yA", fl4(10 * n), &yA));
+  WAP_TRY(scratch(c, "flow_yB", fl4(10 * n), &yB));
+  WAP_TRY(scratch(c, "flow_u", fl4(3 * n), &u));
+  WAP_TRY(scratch(c, "flow_ut", fl4(n), &ut));
   WAP_TRY(scratch(c, "flow_warped", n, &warped));
   WAP_TRY(scratch(c, "flow_kty", 3 * n, &kty));
   WAP_TRY(scratch(c, "flow_kx4", n, &kx4));
@@ -375,9 +384,12 @@ int flow_level(WapContext* c, const int* dims, const float* I1, const float* I2,
   // state: v in V / vB, y in yA / yB (y evolves everywhere and is written back to Y over
   // the ROI at the end); u and ut stay zero outside the ROI
   WAP_HIP(c, wapk_copy(vB, V, 3 * n, st));
-  WAP_HIP(c, wapk_copy(yA, Y, 10 * n, st));
-  WAP_HIP(c, wapk_fill(u, 3 * n, 0.0f, st));
-  WAP_HIP(c, wapk_fill(ut, n, 0.0f, st));
+  if (sto == 0)
+    WAP_HIP(c, wapk_copy(yA, Y, 10 * n, st));
+  else
+    WAP_HIP(c, wapk_to_store(sto, Y, yA, 10 * n, st));
+  WAP_HIP(c, wapk_fill(u, fl4(3 * n), 0.0f, st));
+  WAP_HIP(c, wapk_fill(ut, fl4(n), 0.0f, st));
   WAP_HIP(c, wapk_fill(kty, 3 * n, 0.0f, st));
   WAP_HIP(c, wapk_fill(kx4, n, 0.0f, st));
   int variant = c->flow_variant;
@@ -393,22 +405,22 @@ int flow_level(WapContext* c, const int* dims, const float* I1, const float* I2,
   std::vector<int> ctl_host(4);
   for (int w = 0; w < nwarps; ++w) {
     WAP_HIP(c, wapk_flow_arm(ctl, st));
-    if (w > 0) WAP_HIP(c, wapk_flow_kx4(dims, u, vcur, kx4, st));
+    if (w > 0) WAP_HIP(c, wapk_flow_kx4(sto, dims, u, vcur, kx4, st));
     WAP_HIP(c, wapk_advect(dims, aabb, vcur, I2, 1, -1.0f, warped, st));
-    if (has_roi) WAP_HIP(c, wapk_flow_prep(dims, roi, warped, I1, vcur, u, ut, st));
+    if (has_roi) WAP_HIP(c, wapk_flow_prep(sto, dims, roi, warped, I1, vcur, u, ut, st));
     const int last = max_iter + 1;
     for (int it = 1; it <= last; ++it) {
       const int mode = (it == 1) ? (w == 0 ? 0 : 1) : 2;
       const int resid = (it % 50 == 0);
       const int store = (it % 50 == 49);
-      WAP_HIP(c, wapk_flow_pd(variant, dims, roi, vcur, vnext, ycur, ynext, u, ut, kx4, kty,
+      WAP_HIP(c, wapk_flow_pd(variant, sto, dims, roi, vcur, vnext, ycur, ynext, u, ut, kx4, kty,
                               mode, resid, store, sigma1, lambda, pp, dp, ctl, it, st));
       if (resid)
         WAP_HIP(c, wapk_flow_resid(pp, (int)nblk, dp, (int)nblk, (int)n, tol, ctl, err, st));
       std::swap(vcur, vnext);
       std::swap(ycur, ynext);
     }
-    WAP_HIP(c, wapk_flow_settle(n, last, vcur, vnext, ycur, ynext, ctl, st));
+    WAP_HIP(c, wapk_flow_settle(sto, n, last, vcur, vnext, ycur, ynext, ctl, st));
     if (iters_run) {
       WAP_HIP(c, hipMemcpyAsync(ctl_host.data(), ctl, sizeof(int) * 2, hipMemcpyDeviceToHost, st));
       WAP_HIP(c, hipStreamSynchronize(st));
@@ -416,7 +428,12 @@ int flow_level(WapContext* c, const int* dims, const float* I1, const float* I2,
     }
   }
   if (vcur != V) WAP_HIP(c, wapk_copy(V, vcur, 3 * n, st));
-  if (has_roi) WAP_HIP(c, wapk_copy_box(Y, ycur, dims, 10, lo, hi, st));
+  if (has_roi) {
+    if (sto == 0)
+      WAP_HIP(c, wapk_copy_box(Y, ycur, dims, 10, lo, hi, st));
+    else
+      WAP_HIP(c, wapk_box_from_store(sto, Y, ycur, dims, 10, lo, hi, st));
+  }
   if (iters_run) *iters_run = total;
   return WAP_OK;
 }
@@ -572,6 +589,12 @@ int wap_set_tuning(WapContext* c, const char* key, int value) {
     c->flow_variant = value;
     return WAP_OK;
   }
+  if (strcmp(key, "flow_storage") == 0) {
+    if (value != WAP_F32 && value != WAP_BF16 && value != WAP_F16)
+      return fail(c, WAP_E_ARG, "flow_storage must be 0 (f32), 1 (bf16) or 2 (f16)");
+    c->flow_storage = value;
+    return WAP_OK;
+  }
   if (strcmp(key, "project_unroll") == 0) {
     if (value != 1 && value != 2 && value != 3 && value != 4 && value != 6 && value != 8)
       return fail(c, WAP_E_ARG, "project_unroll must be 1, 2, 3, 4, 6 or 8");
@@ -589,6 +612,10 @@ int wap_get_tuning(const WapContext* c, const char* key, int* value) {
   if (!c || !key || !value) return WAP_E_ARG;
   if (strcmp(key, "flow_variant") == 0) {
     *value = c->flow_variant;
+    return WAP_OK;
+  }
+  if (strcmp(key, "flow_storage") == 0) {
+    *value = c->flow_storage;
     return WAP_OK;
   }
   if (strcmp(key, "project_unroll") == 0) {

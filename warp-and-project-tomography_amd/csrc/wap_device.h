@@ -48,3 +48,41 @@ WAP_DEV float vfetch(const float* __restrict__ vol, int x, int y, int z, int W, 
   z = iclamp(z, 0, D - 1);
   return vol[vidx(x, y, z, W, H)];
 }
+
+// Storage element types of the precision form: values are computed in float and stored
+// as float, IEEE half (round to nearest even) or bfloat16 (round to nearest even, NaN
+// kept quiet).
+struct Bf16 {
+  unsigned short b;
+};
+WAP_DEV float to_f(float x) { return x; }
+WAP_DEV float to_f(_Float16 x) { return (float)x; }
+WAP_DEV float to_f(Bf16 x) { return __uint_as_float((unsigned)x.b << 16); }
+template <class S>
+WAP_DEV S from_f(float v);
+template <>
+WAP_DEV float from_f<float>(float v) {
+  return v;
+}
+template <>
+WAP_DEV _Float16 from_f<_Float16>(float v) {
+  return (_Float16)v;
+}
+template <>
+WAP_DEV Bf16 from_f<Bf16>(float v) {
+  unsigned u = __float_as_uint(v);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return Bf16{(unsigned short)((u >> 16) | 0x40u)};
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return Bf16{(unsigned short)(u >> 16)};
+}
+
+// Element access at a float-scaled byte offset t4 (= 4 x element index) from a uniform
+// plane base: the offset of an S element is t4 * sizeof(S) / 4.
+template <class S>
+WAP_DEV float lds(const S* __restrict__ base, unsigned t4) {
+  return to_f(*(const S*)((const char*)base + (sizeof(S) == 4 ? t4 : t4 >> 1)));
+}
+template <class S>
+WAP_DEV void sts(S* __restrict__ base, unsigned t4, float v) {
+  *(S*)((char*)base + (sizeof(S) == 4 ? t4 : t4 >> 1)) = from_f<S>(v);
+}
