@@ -40,6 +40,14 @@ def parse():
     p.add_argument("--flow-iters", type=int, default=50)
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-sample-seconds", type=float, default=20.0)
+    p.add_argument("--profile-pd", action="store_true",
+                   help="run only the profiling workload of the PD iteration: one 8-iteration "
+                        "flow solve at full size, then a 13-plane copy of known size that "
+                        "calibrates the HBM byte counters (use under rocprofv3)")
+    p.add_argument("--pd-form", type=int, default=-1,
+                   help="with --profile-pd: the PD iteration form (-1 = autotuned)")
+    p.add_argument("--no-forms", action="store_true",
+                   help="skip the precision-form step (half storage of the flow state)")
     return p.parse_args()
 
 
@@ -77,10 +85,15 @@ def make_workload(ctx, W, L, S, NDET, torch, dev):
     return vols, views, angles, sino, sd
 
 
-def roofline_pd(ctx, torch, dev, L):
-    """Average duration of one flow primal-dual iteration at full resolution, from two
-    solves that differ only in the iteration count, timed with events on the stream the
-    kernels run on (torch's current stream)."""
+ZM_NAME = "k_flow_zm (single-pass primal-dual iteration of the L1 flow)"
+TWO_NAME = "k_flow_primal + k_flow_dual (primal-dual iteration of the L1 flow)"
+
+
+def pd_iteration_ms(ctx, torch, dev, L, storage=0):
+    """Average duration of one flow primal-dual iteration at full resolution (the form the
+    autotuner picks), from two solves that differ only in the iteration count, timed with
+    events on the stream the kernels run on (torch's current stream)."""
+    ctx.set_tuning("flow_storage", storage)
     I1 = torch.rand((L, L, L), device=dev)
     I2 = torch.rand((L, L, L), device=dev)
     V = torch.zeros((3, L, L, L), device=dev)
@@ -102,14 +115,35 @@ def roofline_pd(ctx, torch, dev, L):
     run(5)
     t_lo = min(run(10) for _ in range(2))
     t_hi = min(run(60) for _ in range(2))
-    per_ms = (t_hi - t_lo) / 50.0
+    variant = ctx.get_tuning("flow_variant_used")
+    ctx.set_tuning("flow_storage", 0)
+    return (t_hi - t_lo) / 50.0, variant
+
+
+def measured_traffic(variant, L):
+    """HBM bytes per launch of the PD iteration from the committed rocprofv3 counter
+    summary (FETCH_SIZE and WRITE_SIZE passes scaled by a copy of known size), when it
+    was taken for this same form and size; otherwise None."""
+    path = os.path.join(ROOT, "profiles", f"flow_pd_{L}_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith(f"form_{variant}_traffic_bytes_per_launch:"):
+            return int(float(line.split(":", 1)[1]))
+    return None
+
+
+def roofline_pd(ctx, torch, dev, L):
+    per_ms, variant = pd_iteration_ms(ctx, torch, dev, L)
     n = L * L * L
-    # compulsory bytes per iteration: read v (3), y (10), u (3), ut (1); write v (3), y (10)
-    algo = 30 * 4 * n
+    # algorithmic bytes per voxel-iteration: read v (3), v_old (3), y (10), u (4);
+    # write v (3), y (10) = 33 floats = 132 B
+    algo = 132 * n
     ach = algo / (per_ms * 1e-3) / 1e9
     return {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
-            "kernel": "k_flow_pd (one fused primal-dual iteration)",
+            "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
+            "traffic": measured_traffic(variant, L),
+            "kernel": (TWO_NAME if variant == 0 else ZM_NAME) + f", form {variant}",
             "ms_per_launch": round(per_ms, 4), "bytes_per_launch": algo}
 
 
@@ -194,6 +228,23 @@ def main():
     dev = "cuda:0"
     L, S, NDET = a.size, a.states, a.det
     ctx = W.Context(0)
+    if a.profile_pd:
+        ctx.set_tuning("flow_variant", a.pd_form)
+        I1 = torch.rand((L, L, L), device=dev)
+        I2 = torch.rand((L, L, L), device=dev)
+        V = torch.zeros((3, L, L, L), device=dev)
+        Y = torch.zeros((10, L, L, L), device=dev)
+        ctx.flow_solve(I1, I2, V, Y, 0.0, 0.05, 8, 1, [0, L, 0, L, 0, L], [8] * 6)
+        torch.cuda.synchronize()
+        src = torch.rand(13 * L ** 3, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"profile_pd": True, "size": L,
+                          "form": ctx.get_tuning("flow_variant_used"),
+                          "copy_bytes_each_way": src.numel() * 4}))
+        ctx.close()
+        return
     print(f"[bench] generating {L}^3 x {S} states, {NDET}^2 detector", file=sys.stderr,
           flush=True)
     vols_truth, views, angles, sino, sd = make_workload(ctx, W, L, S, NDET, torch, dev)
@@ -231,6 +282,28 @@ def main():
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     assert torch.isfinite(vols).all() and float(vols.max()) > 0.0
+    forms = None
+    if not a.no_forms:
+        # the precision form: the same outer iteration with the flow state stored in half
+        # (its one-off autotuning of the half-storage forms, a few hundred short launches,
+        # falls inside this step; the scratch is already allocated by the float steps)
+        ctx.set_tuning("flow_storage", 2)
+        vols.zero_()
+        flows.zero_()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        ctx.iteration(cfg, vols, flows, sino, ws, we, ft, mask)
+        torch.cuda.synchronize()
+        dt16 = time.perf_counter() - t1
+        ctx.set_tuning("flow_storage", 0)
+        pd16, v16 = pd_iteration_ms(ctx, torch, dev, L, storage=2)
+        forms = {"precision": {"flow_storage": "f16 (duals, gradient, ut)",
+                               "ms_per_step": round(dt16 * 1000.0, 1),
+                               "speedup_vs_plain": round((dt / a.steps) / dt16, 3),
+                               "pd_iteration_ms": round(pd16, 4), "pd_form": v16,
+                               "error_vs_fp64": "flow <= 5e-4, volumes <= 1e-4 rel L2 "
+                                                "(tests/test_gpu_precision.py)"}}
+        print("[bench] precision-form step done", file=sys.stderr, flush=True)
     del flows, vols
     torch.cuda.empty_cache()
     roof = roofline_pd(ctx, torch, dev, L)
@@ -261,6 +334,7 @@ def main():
         "projector": proj_stats,
         "roofline": roof,
         "cpu_baseline": cpu,
+        "forms": forms,
     }
     print(json.dumps(out))
     ctx.close()
