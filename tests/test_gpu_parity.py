@@ -323,3 +323,16 @@ def test_projector_unroll_identical(wctx, oracle, W, unroll):
             np.testing.assert_array_equal(P[k], a)
     finally:
         wctx.set_tuning("project_unroll", prev)
+
+
+def test_advect_nonfinite_velocity(wctx, oracle):
+    """A velocity field with an infinite value: the cell-centre shortcut is off and the
+    output is the oracle's (the grid clamp absorbs the non-finite positions)."""
+    shape = (14, 15, 16)
+    src = helpers.blob_volume(shape, 31)
+    vel = helpers.smooth_flow(shape, 31, 1.2)
+    vel[1, 5, 6, 7] = np.inf
+    out = torch.zeros(shape, device=DEV)
+    wctx.advect(T(vel), T(src), -1.0, out)
+    ref = oracle.advect(vel, src, -1.0)
+    np.testing.assert_array_equal(H(out), ref)

@@ -102,9 +102,24 @@ WAP_DEV float sample_taps(const float* __restrict__ img, const Taps& T) {
   return (mn < lo) ? lo : mn;
 }
 
+// flag = 1 if any of the n values is not finite (the flag is cleared beforehand)
+__global__ __launch_bounds__(256) void k_nonfinite(const float* __restrict__ a, size_t n,
+                                                   int* __restrict__ flag) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n && !isfinite(a[t])) *flag = 1;
+}
+
+__global__ void k_clear_flag(int* flag) { *flag = 0; }
+
+// vel_bad: device flag that is 0 when every velocity value is finite. Then the velocity
+// sample at a cell centre whose grid position is exactly integral is the grid value
+// itself: Catmull-Rom at zero offset returns the centre tap, and the 8-cell clamp keeps
+// it (a zero of either sign leaves p + dt*v unchanged), so the 3 x 64-tap evaluation is
+// skipped with bit-identical results.
 __global__ __launch_bounds__(256) void k_advect(AdvGrid G, const float* __restrict__ vel,
                                                 const float* __restrict__ src, int nc, float dt,
-                                                float* __restrict__ out) {
+                                                float* __restrict__ out,
+                                                const int* __restrict__ vel_bad) {
   const size_t n = (size_t)G.W * G.H * G.D;
   const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
@@ -123,8 +138,13 @@ __global__ __launch_bounds__(256) void k_advect(AdvGrid G, const float* __restri
   float g[3], s[3], p2[3], p0[3], q[3];
   clamp_grid(G, p, g);
   Taps T = make_taps(G, g);
+  if (T.dx == 0.0f && T.dy == 0.0f && T.dz == 0.0f && *vel_bad == 0) {
 #pragma unroll
-  for (int c = 0; c < 3; ++c) s[c] = sample_taps(vel + c * n, T);
+    for (int c = 0; c < 3; ++c) s[c] = vel[c * n + T.c000];
+  } else {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) s[c] = sample_taps(vel + c * n, T);
+  }
 #pragma unroll
   for (int a = 0; a < 3; ++a) p2[a] = p[a] + tdt * s[a];
   clamp_grid(G, p2, g);
@@ -142,15 +162,19 @@ __global__ __launch_bounds__(256) void k_advect(AdvGrid G, const float* __restri
 
 }  // namespace
 
+// flag: one int of device scratch for the finiteness check of vel
 hipError_t wapk_advect(const int* dims, const float* aabb, const float* vel, const float* src,
-                       int nc, float dt, float* out, hipStream_t s) {
+                       int nc, float dt, float* out, int* flag, hipStream_t s) {
   AdvGrid G;
   G.W = dims[0];
   G.H = dims[1];
   G.D = dims[2];
   for (int a = 0; a < 6; ++a) G.bb[a] = aabb[a];
   const size_t n = (size_t)G.W * G.H * G.D;
+  hipLaunchKernelGGL(k_clear_flag, dim3(1), dim3(1), 0, s, flag);
+  hipLaunchKernelGGL(k_nonfinite, dim3((unsigned)((3 * n + 255) / 256)), dim3(256), 0, s, vel,
+                     3 * n, flag);
   hipLaunchKernelGGL(k_advect, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, G, vel, src,
-                     nc, dt, out);
+                     nc, dt, out, flag);
   return hipGetLastError();
 }

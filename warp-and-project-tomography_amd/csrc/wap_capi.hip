@@ -32,7 +32,7 @@ hipError_t wapk_scatter_finish(const double*, const double*, size_t, double, flo
 hipError_t wapk_bp_voxel(float*, const int*, const float*, const WapView*, int, int, double,
                          float, const int*, const int*, int, hipStream_t);
 hipError_t wapk_advect(const int*, const float*, const float*, const float*, int, float, float*,
-                       hipStream_t);
+                       int*, hipStream_t);
 int wapk_flow_variants();
 size_t wapk_flow_blocks(int, const int*);
 size_t wapk_flow_elem(int);
@@ -141,6 +141,10 @@ int scratch(WapContext* c, const std::string& name, size_t count, T** out) {
 }
 
 bool dims_ok(const int* d) { return d && d[0] > 0 && d[1] > 0 && d[2] > 0; }
+
+// advection with the context's finiteness flag (one int of scratch)
+int advect_impl(WapContext* c, const int* dims, const float* aabb, const float* vel,
+                const float* src, int nc, float dt, float* out, hipStream_t st);
 size_t nvox(const int* d) { return (size_t)d[0] * d[1] * d[2]; }
 hipStream_t S(void* s) { return (hipStream_t)s; }
 
@@ -306,6 +310,14 @@ int blur_impl(WapContext* c, const float* src, const int* dims, int nc, float si
 // ---------------------------------------------------------------------------------
 // optical flow: one scale (the warp/iteration loop)
 // ---------------------------------------------------------------------------------
+int advect_impl(WapContext* c, const int* dims, const float* aabb, const float* vel,
+                const float* src, int nc, float dt, float* out, hipStream_t st) {
+  int* flag;
+  WAP_TRY(scratch(c, "advect_flag", 1, &flag));
+  WAP_HIP(c, wapk_advect(dims, aabb, vel, src, nc, dt, out, flag, st));
+  return WAP_OK;
+}
+
 // Autotuner of the primal-dual iteration form: on the first flow level of a grid size,
 // time each candidate form (all give the same floats) on the level's own buffers and
 // keep the fastest for that size. The timed launches read (vcur, ycur) and write
@@ -406,7 +418,7 @@ int flow_level(WapContext* c, const int* dims, const float* I1, const float* I2,
   for (int w = 0; w < nwarps; ++w) {
     WAP_HIP(c, wapk_flow_arm(ctl, st));
     if (w > 0) WAP_HIP(c, wapk_flow_kx4(sto, dims, u, vcur, kx4, st));
-    WAP_HIP(c, wapk_advect(dims, aabb, vcur, I2, 1, -1.0f, warped, st));
+    WAP_TRY(advect_impl(c, dims, aabb, vcur, I2, 1, -1.0f, warped, st));
     if (has_roi) WAP_HIP(c, wapk_flow_prep(sto, dims, roi, warped, I1, vcur, u, ut, st));
     const int last = max_iter + 1;
     for (int it = 1; it <= last; ++it) {
@@ -549,14 +561,14 @@ int warp_project_impl(WapContext* c, const WapLoopConfig* L, float* vols, const 
   float* vol = vols + (size_t)frame * n;
   const int nf = L->nstates;
   const int fsel = rtime < 0.0f ? std::max(0, frame - 1) : std::min(nf - 1, frame);
-  WAP_HIP(c, wapk_advect(L->dims, aabb, flows + (size_t)fsel * 3 * n, vol, 1, rtime, warped, st));
+  WAP_TRY(advect_impl(c, L->dims, aabb, flows + (size_t)fsel * 3 * n, vol, 1, rtime, warped, st));
   WAP_HIP(c, wapk_project(warped, 0, L->dims, L->sd, c->views + view, 1, L->nu, L->nv, nullptr,
                           nullptr, nullptr, b, nullptr, corr, L->alpha, c->project_unroll, st));
   WAP_HIP(c, wapk_fill(delta, n, 0.0f, st));
   const int lo[3] = {0, 0, 0};
   WAP_HIP(c, wapk_bp_voxel(delta, L->dims, corr, c->views + view, L->nu, L->nv, L->alpha, 0.0f,
                            lo, L->dims, 1, st));
-  WAP_HIP(c, wapk_advect(L->dims, aabb, flows + (size_t)std::max(0, frame - 1) * 3 * n, delta, 1,
+  WAP_TRY(advect_impl(c, L->dims, aabb, flows + (size_t)std::max(0, frame - 1) * 3 * n, delta, 1,
                          -rtime, back, st));
   WAP_HIP(c, wapk_wap_update(vol, back, L->alpha, L->gmax, mask, n, st));
   return WAP_OK;
@@ -742,7 +754,7 @@ int wap_advect(WapContext* c, const int* dims, const float* aabb, const float* v
     return fail(c, WAP_E_ARG, "wap_advect: every extent must be at least 3");
   if (!(aabb[1] > aabb[0] && aabb[3] > aabb[2] && aabb[5] > aabb[4]))
     return fail(c, WAP_E_ARG, "wap_advect: empty bounding box");
-  WAP_HIP(c, wapk_advect(dims, aabb, vel, src, nc, dt, out, S(stream)));
+  WAP_TRY(advect_impl(c, dims, aabb, vel, src, nc, dt, out, S(stream)));
   return WAP_OK;
 }
 
