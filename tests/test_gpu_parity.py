@@ -336,3 +336,44 @@ def test_advect_nonfinite_velocity(wctx, oracle):
     wctx.advect(T(vel), T(src), -1.0, out)
     ref = oracle.advect(vel, src, -1.0)
     np.testing.assert_array_equal(H(out), ref)
+
+
+@pytest.mark.parametrize("mapping", [0, 2, 4, 8, 16])
+@pytest.mark.parametrize("parallel", [False, True])
+def test_projector_mappings_identical(wctx, oracle, W, mapping, parallel):
+    """Every lanes-to-rays mapping of the forward projector (patches, rays walking
+    consecutive samples on several lanes) gives the oracle's floats, with the SART
+    correction fused."""
+    shape, nu, nv = (36, 30, 40), 53, 47
+    vol = helpers.blob_volume(shape, 6)
+    geo = (helpers.parallel_geometry if parallel else helpers.cone_geometry)(W, shape, nu, nv)
+    angles = [0.0, 27.0, 45.0, 90.0, 133.0, 250.0, 300.5]
+    views = W.views(geo, angles)
+    wctx.upload_views(views)
+    prev = wctx.get_tuning("project_mapping")
+    wctx.set_tuning("project_mapping", mapping)
+    try:
+        n = len(angles)
+        proj = torch.zeros((n, nv, nu), device=DEV)
+        rl = torch.zeros_like(proj)
+        wt = torch.zeros_like(proj)
+        wctx.forward_project(T(vol), SD, 0, n, nu, nv, proj, rl, wt)
+        P, Rl, Wt = H(proj), H(rl), H(wt)
+        for k in range(n):
+            a, r, c = oracle.forward_project(vol, views[k], nu, nv, SD)
+            np.testing.assert_array_equal(Wt[k], c)
+            np.testing.assert_array_equal(Rl[k], r)
+            np.testing.assert_array_equal(P[k], a)
+        # fused SART correction on one view
+        rng = np.random.default_rng(3)
+        b = (rng.random((nv, nu)) * 4).astype(np.float32)
+        pyv = (rng.random((nv, nu)) * 0.1).astype(np.float32)
+        corr = torch.zeros((nv, nu), device=DEV)
+        tpy = T(pyv)
+        wctx.project_correct(T(vol), SD, 3, nu, nv, T(b), tpy, 0.2, corr)
+        a, r, _ = oracle.forward_project(vol, views[3], nu, nv, SD)
+        co, pyo = oracle.sart_correct(b, a, r, pyv, 0.2)
+        np.testing.assert_array_equal(H(corr), co)
+        np.testing.assert_array_equal(H(tpy), pyo)
+    finally:
+        wctx.set_tuning("project_mapping", prev)

@@ -362,6 +362,145 @@ __global__ __launch_bounds__(256) void k_project(
 }
 
 // ---------------------------------------------------------------------------------
+// forward projection with R rays per wave, each on G = 64/R lanes that take G
+// consecutive samples of their ray at a time (R = 1: one ray per wave). When rays run
+// along x, the volume's fastest axis, a gather of one wave then touches one or two cache
+// lines instead of dozens. Bit-identical to k_project: each lane replays the sequential
+// t += sd / pos += step additions from its group's chunk start, and each ray's samples
+// are added to its sum one by one in order (through lane shuffles), as the
+// one-sample-at-a-time march does.
+// grid: x = tiles of R x 4 pixels (u x v; wave w takes row 4*tile_v + w), y = view
+// ---------------------------------------------------------------------------------
+template <int R>
+__global__ __launch_bounds__(256) void k_project_rays(
+    const float* __restrict__ vol, size_t vol_stride, int W, int H, int D, float sd,
+    const WapView* __restrict__ views, int nu, int nv, float* __restrict__ proj,
+    float* __restrict__ raylen, float* __restrict__ weight, const float* __restrict__ b,
+    float* __restrict__ py, float* __restrict__ corr, double alpha) {
+  constexpr int G = 64 / R;
+  const int lane = (int)(threadIdx.x & 63u), wave = (int)(threadIdx.x >> 6);
+  const int grp = lane / G, g = lane % G, base = grp * G;
+  const int tilesU = (nu + R - 1) / R;
+  const int i = (int)(blockIdx.x % (unsigned)tilesU) * R + grp;
+  const int j = (int)(blockIdx.x / (unsigned)tilesU) * 4 + wave;
+  const bool pix_ok = i < nu && j < nv;
+  const int view = blockIdx.y;
+  const WapView& v = views[view];
+  const float* V = vol + vol_stride * view;
+  // every lane of the wave runs the march (shuffles need them all); pixels past the
+  // detector edge trace the edge pixel and store nothing
+  const RaySeg r = produce_ray(v, W, H, D, min(i, nu - 1), min(j, nv - 1));
+  const float tn = r.tn, tf = r.tf;
+  float density = 0.0f, sw = 0.0f, len = 0.0f;
+  bool done = !(tn < tf);
+  float step[3], tMax[3] = {0.0f, 0.0f, 0.0f}, tDel[3] = {0.0f, 0.0f, 0.0f};
+#pragma unroll
+  for (int a = 0; a < 3; ++a) step[a] = r.u[a] * sd;
+  const float span = (float)(W + H + D);
+  const int cap = (int)(8.0f * (float)(W + H + D) / sd) + 16;
+  float t1 = tn;
+  int visits = 1, segments = 0;
+  if (!done) {
+    len = tf - tn;
+    float p1[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) p1[a] = r.z[a] + r.u[a] * t1;
+    sw += 1.0f;
+    density += trilinear32(V, p1[0], p1[1], p1[2], W, H, D) * sd;
+    const int bs[3] = {W, H, D};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      if (r.u[a] == 0.0f) {
+        tMax[a] = tf;
+      } else if (r.u[a] < 0.0f) {
+        const int x = ((int)ceilf(p1[a] - 0.5f) - 1) / bs[a];
+        tMax[a] = ((float)(x * bs[a]) + 0.5f - r.z[a]) / r.u[a];
+        tDel[a] = (float)(-bs[a]) / r.u[a];
+      } else {
+        const int x = (int)floorf(p1[a] - 0.5f) / bs[a];
+        tMax[a] = ((float)((x + 1) * bs[a]) + 0.5f - r.z[a]) / r.u[a];
+        tDel[a] = (float)bs[a] / r.u[a];
+      }
+    }
+  }
+  bool in_seg = false;
+  float t = 0.0f, px = 0.0f, py_ = 0.0f, pz = 0.0f, tb = 0.0f;
+  const unsigned long long gmask = G == 64 ? ~0ull : (((1ull << G) - 1ull) << base);
+  while (__any(!done)) {
+    if (!done && !in_seg) {
+      if (t1 < tf && segments < 16) {
+        ++segments;
+        tb = mmin(mmin(tMax[0], tMax[1]), tMax[2]);
+        if (tb > tf) tb = tf;
+        if (tn - t1 > span) t1 = tn;
+        t = ceilf((t1 - tn) / sd) * sd + tn;
+        px = r.z[0] + r.u[0] * t;
+        py_ = r.z[1] + r.u[1] * t;
+        pz = r.z[2] + r.u[2] * t;
+        in_seg = true;
+      } else {
+        done = true;
+      }
+    }
+    // this lane's sample: g additions from the chunk start
+    float tg = t, xg = px, yg = py_, zg = pz;
+#pragma unroll
+    for (int q = 0; q < G - 1; ++q) {
+      const bool adv = q < g;
+      tg = adv ? tg + sd : tg;
+      xg = adv ? xg + step[0] : xg;
+      yg = adv ? yg + step[1] : yg;
+      zg = adv ? zg + step[2] : zg;
+    }
+    const bool ok = !done && in_seg && tg < tb && visits + g < cap;
+    const float val = trilinear32(V, xg, yg, zg, W, H, D) * sd;
+    const int count = __popcll(__ballot(ok) & gmask);
+#pragma unroll
+    for (int q = 0; q < G; ++q) {
+      const float vq = __shfl(val, base + q, 64);
+      density += q < count ? vq : 0.0f;
+    }
+    sw += (float)count;
+    visits += count;
+    const float tl = __shfl(tg, base + G - 1, 64), xl = __shfl(xg, base + G - 1, 64);
+    const float yl = __shfl(yg, base + G - 1, 64), zl = __shfl(zg, base + G - 1, 64);
+    if (!done && in_seg) {
+      if (count == G) {
+        t = tl + sd;
+        px = xl + step[0];
+        py_ = yl + step[1];
+        pz = zl + step[2];
+      } else {
+        if (tMax[0] < tMax[1]) {
+          if (tMax[0] < tMax[2])
+            tMax[0] += tDel[0];
+          else
+            tMax[2] += tDel[2];
+        } else {
+          if (tMax[1] < tMax[2])
+            tMax[1] += tDel[1];
+          else
+            tMax[2] += tDel[2];
+        }
+        t1 = tb;
+        in_seg = false;
+      }
+    }
+  }
+  if (g != 0 || !pix_ok) return;
+  const size_t k = (size_t)view * nu * nv + (size_t)j * nu + i;
+  if (proj) proj[k] = density;
+  if (raylen) raylen[k] = len;
+  if (weight) weight[k] = sw;
+  if (corr) {
+    const float y = py ? py[k] : 0.0f;
+    const float c = (b[k] - density - y) / (len + 1.0f);
+    corr[k] = c;
+    if (py) py[k] = (float)((double)y + alpha * (double)c);
+  }
+}
+
+// ---------------------------------------------------------------------------------
 // matched ray-driven scatter: acc[v] += corr[ray]*w, wacc[v] += w over the exact
 // forward sample set (FP64 atomics, so the sum does not depend on ray order to
 // within double rounding)
@@ -472,12 +611,30 @@ __global__ __launch_bounds__(256) void k_bp_voxel(float* __restrict__ vol, int W
 // ---------------------------------------------------------------------------------
 // launchers (called from wap_capi.cpp)
 // ---------------------------------------------------------------------------------
+// mapping: 0 = 8x8 pixel patches per wave with `unroll` samples per trip (rays along z);
+// 2, 4, 8 or 16 = that many rays per wave, each on 64/mapping lanes of consecutive
+// samples (rays with a large x component)
 hipError_t wapk_project(const float* vol, size_t vol_stride, const int* dims, float sd,
                         const WapView* views_dev, int nviews, int nu, int nv, float* proj,
                         float* raylen, float* weight, const float* b, float* py, float* corr,
-                        double alpha, int unroll, hipStream_t s) {
+                        double alpha, int unroll, int mapping, hipStream_t s) {
   // 32-bit byte offsets inside one volume
   if ((size_t)dims[0] * dims[1] * dims[2] >= (1ull << 30)) return hipErrorInvalidValue;
+  if (mapping > 0) {
+    const unsigned tiles = (unsigned)(((nu + mapping - 1) / mapping) * ((nv + 3) / 4));
+#define WAP_PR(R)                                                                              \
+  hipLaunchKernelGGL(k_project_rays<R>, dim3(tiles, nviews), dim3(256), 0, s, vol, vol_stride, \
+                     dims[0], dims[1], dims[2], sd, views_dev, nu, nv, proj, raylen, weight, b, \
+                     py, corr, alpha)
+    switch (mapping) {
+      case 2: WAP_PR(2); break;
+      case 4: WAP_PR(4); break;
+      case 16: WAP_PR(16); break;
+      default: WAP_PR(8); break;
+    }
+#undef WAP_PR
+    return hipGetLastError();
+  }
   const unsigned tiles = ((nu + 15) / 16) * ((nv + 15) / 16);
 #define WAP_PJ(U)                                                                              \
   hipLaunchKernelGGL(k_project<U>, dim3(tiles, nviews), dim3(256), 0, s, vol, vol_stride,     \

@@ -23,7 +23,7 @@
 
 // kernel launchers (projector.hip, advect.hip, flow.hip, imageops.hip)
 hipError_t wapk_project(const float*, size_t, const int*, float, const WapView*, int, int, int,
-                        float*, float*, float*, const float*, float*, float*, double, int,
+                        float*, float*, float*, const float*, float*, float*, double, int, int,
                         hipStream_t);
 hipError_t wapk_scatter(const float*, const int*, float, const WapView*, int, int, int, double*,
                         double*, hipStream_t);
@@ -98,6 +98,8 @@ struct WapContext {
   int flow_storage = 0;  // storage of the flow duals, gradient and ut (WapDType)
   int flow_last = 0;                              // form used by the last flow level
   int project_unroll = 2;                         // ray samples per march trip
+  int project_mapping = -1;  // lanes-to-rays mapping of the projector; -1 = per view
+  std::vector<WapView> host_views;                // host copy of the view table
 };
 
 namespace {
@@ -310,6 +312,44 @@ int blur_impl(WapContext* c, const float* src, const int* dims, int nc, float si
 // ---------------------------------------------------------------------------------
 // optical flow: one scale (the warp/iteration loop)
 // ---------------------------------------------------------------------------------
+// Lanes-to-rays mapping of the forward projector for one view: rays with a large x
+// component (the volume's fastest axis) run faster with 8 rays per wave walking 8
+// consecutive samples each; rays close to z with 8x8 pixel patches per wave.
+int project_mapping(const WapContext* c, int view, const int* dims) {
+  if (c->project_mapping >= 0) return c->project_mapping;
+  const WapView& v = c->host_views[view];
+  float d[3];
+  if (v.parallel) {
+    for (int a = 0; a < 3; ++a) d[a] = v.dir[a] * v.m2v[5 * a];
+  } else {
+    for (int a = 0; a < 3; ++a) d[a] = 0.5f * (float)(dims[a] - 1) - v.eye[a] * v.m2v[5 * a];
+  }
+  const float len = sqrtf(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+  return (len > 0.0f && fabsf(d[0]) >= 0.4f * len) ? 8 : 0;
+}
+
+// Forward projection of views [view0, view0 + nviews) (consecutive volumes vol_stride
+// apart), launched in runs of views that share a mapping.
+int project_views(WapContext* c, const float* vol, size_t vol_stride, const int* dims, float sd,
+                  int view0, int nviews, int nu, int nv, float* proj, float* raylen,
+                  float* weight, const float* b, float* py, float* corr, double alpha,
+                  hipStream_t st) {
+  const size_t nd = (size_t)nu * nv;
+  int k = 0;
+  while (k < nviews) {
+    const int m = project_mapping(c, view0 + k, dims);
+    int e = k + 1;
+    while (e < nviews && project_mapping(c, view0 + e, dims) == m) ++e;
+    WAP_HIP(c, wapk_project(vol + vol_stride * k, vol_stride, dims, sd, c->views + view0 + k,
+                            e - k, nu, nv, proj ? proj + nd * k : nullptr,
+                            raylen ? raylen + nd * k : nullptr,
+                            weight ? weight + nd * k : nullptr, b, py, corr, alpha,
+                            c->project_unroll, m, st));
+    k = e;
+  }
+  return WAP_OK;
+}
+
 int advect_impl(WapContext* c, const int* dims, const float* aabb, const float* vel,
                 const float* src, int nc, float dt, float* out, hipStream_t st) {
   int* flag;
@@ -536,9 +576,9 @@ int sart_prox_impl(WapContext* c, const WapLoopConfig* L, float* vol, const floa
   for (int s = 0; s < L->nb_sart; ++s) {
     for (int q = 0; q < nw; ++q) {
       const int view = order[q];
-      WAP_HIP(c, wapk_project(vol, 0, L->dims, L->sd, c->views + view, 1, L->nu, L->nv, nullptr,
-                              nullptr, nullptr, sino + (size_t)view * nd,
-                              py + (size_t)(view - first) * nd, corr, L->alpha, c->project_unroll, st));
+      WAP_TRY(project_views(c, vol, 0, L->dims, L->sd, view, 1, L->nu, L->nv, nullptr,
+                            nullptr, nullptr, sino + (size_t)view * nd,
+                            py + (size_t)(view - first) * nd, corr, L->alpha, st));
       WAP_HIP(c, wapk_bp_voxel(vol, L->dims, corr, c->views + view, L->nu, L->nv, L->alpha,
                                L->gmax, lo, hi, 0, st));
     }
@@ -562,8 +602,8 @@ int warp_project_impl(WapContext* c, const WapLoopConfig* L, float* vols, const 
   const int nf = L->nstates;
   const int fsel = rtime < 0.0f ? std::max(0, frame - 1) : std::min(nf - 1, frame);
   WAP_TRY(advect_impl(c, L->dims, aabb, flows + (size_t)fsel * 3 * n, vol, 1, rtime, warped, st));
-  WAP_HIP(c, wapk_project(warped, 0, L->dims, L->sd, c->views + view, 1, L->nu, L->nv, nullptr,
-                          nullptr, nullptr, b, nullptr, corr, L->alpha, c->project_unroll, st));
+  WAP_TRY(project_views(c, warped, 0, L->dims, L->sd, view, 1, L->nu, L->nv, nullptr,
+                        nullptr, nullptr, b, nullptr, corr, L->alpha, st));
   WAP_HIP(c, wapk_fill(delta, n, 0.0f, st));
   const int lo[3] = {0, 0, 0};
   WAP_HIP(c, wapk_bp_voxel(delta, L->dims, corr, c->views + view, L->nu, L->nv, L->alpha, 0.0f,
@@ -607,6 +647,12 @@ int wap_set_tuning(WapContext* c, const char* key, int value) {
     c->flow_storage = value;
     return WAP_OK;
   }
+  if (strcmp(key, "project_mapping") == 0) {
+    if (value != -1 && value != 0 && value != 2 && value != 4 && value != 8 && value != 16)
+      return fail(c, WAP_E_ARG, "project_mapping must be -1, 0, 2, 4, 8 or 16");
+    c->project_mapping = value;
+    return WAP_OK;
+  }
   if (strcmp(key, "project_unroll") == 0) {
     if (value != 1 && value != 2 && value != 3 && value != 4 && value != 6 && value != 8)
       return fail(c, WAP_E_ARG, "project_unroll must be 1, 2, 3, 4, 6 or 8");
@@ -628,6 +674,10 @@ int wap_get_tuning(const WapContext* c, const char* key, int* value) {
   }
   if (strcmp(key, "flow_storage") == 0) {
     *value = c->flow_storage;
+    return WAP_OK;
+  }
+  if (strcmp(key, "project_mapping") == 0) {
+    *value = c->project_mapping;
     return WAP_OK;
   }
   if (strcmp(key, "project_unroll") == 0) {
@@ -688,6 +738,7 @@ int wap_upload_views(WapContext* c, const WapView* hv, int n) {
       return fail(c, WAP_E_NOMEM, "view table allocation failed");
   }
   c->nviews = n;
+  c->host_views.assign(hv, hv + n);
   WAP_HIP(c, hipMemcpy(c->views, hv, sizeof(WapView) * n, hipMemcpyHostToDevice));
   return WAP_OK;
 }
@@ -698,8 +749,8 @@ int wap_forward_project(WapContext* c, const float* vol, size_t vol_stride, cons
   if (!c || !vol || !proj || !dims_ok(dims) || nu <= 0 || nv <= 0 || !(sd > 0.0f))
     return fail(c, WAP_E_ARG, "wap_forward_project: invalid arguments");
   WAP_TRY(check_views(c, view0, nviews));
-  WAP_HIP(c, wapk_project(vol, vol_stride, dims, sd, c->views + view0, nviews, nu, nv, proj,
-                          raylen, weight, nullptr, nullptr, nullptr, 0.0, c->project_unroll, S(stream)));
+  WAP_TRY(project_views(c, vol, vol_stride, dims, sd, view0, nviews, nu, nv, proj,
+                        raylen, weight, nullptr, nullptr, nullptr, 0.0, S(stream)));
   return WAP_OK;
 }
 
@@ -709,8 +760,8 @@ int wap_project_correct(WapContext* c, const float* vol, const int* dims, float 
   if (!c || !vol || !b || !corr || !dims_ok(dims) || nu <= 0 || nv <= 0 || !(sd > 0.0f))
     return fail(c, WAP_E_ARG, "wap_project_correct: invalid arguments");
   WAP_TRY(check_views(c, view, 1));
-  WAP_HIP(c, wapk_project(vol, 0, dims, sd, c->views + view, 1, nu, nv, proj, raylen, nullptr, b,
-                          py, corr, alpha, c->project_unroll, S(stream)));
+  WAP_TRY(project_views(c, vol, 0, dims, sd, view, 1, nu, nv, proj, raylen, nullptr, b,
+                        py, corr, alpha, S(stream)));
   return WAP_OK;
 }
 
