@@ -100,6 +100,12 @@ struct WapContext {
   int project_unroll = 2;                         // ray samples per march trip
   int project_mapping = -1;  // lanes-to-rays mapping of the projector; -1 = per view
   std::vector<WapView> host_views;                // host copy of the view table
+  // concurrent flow solves of independent state pairs: worker streams (default
+  // priority) and their events; `lane` selects the scratch set of the running worker
+  int flow_streams = 4;
+  std::vector<hipStream_t> workers;
+  std::vector<hipEvent_t> worker_events;
+  int lane = 0;
 };
 
 namespace {
@@ -125,7 +131,7 @@ int fail(WapContext* c, int code, const std::string& msg) {
 // Named scratch that grows on demand (reallocation synchronises the device).
 template <class T>
 int scratch(WapContext* c, const std::string& name, size_t count, T** out) {
-  DevBuf& b = c->bufs[name];
+  DevBuf& b = c->bufs[c->lane ? name + "@" + std::to_string(c->lane) : name];
   const size_t need = std::max<size_t>(count * sizeof(T), 256);
   if (b.bytes < need) {
     if (b.ptr) {
@@ -346,6 +352,20 @@ int project_views(WapContext* c, const float* vol, size_t vol_stride, const int*
                             weight ? weight + nd * k : nullptr, b, py, corr, alpha,
                             c->project_unroll, m, st));
     k = e;
+  }
+  return WAP_OK;
+}
+
+int ensure_workers(WapContext* c, int k) {
+  while ((int)c->workers.size() < k) {
+    hipStream_t s;
+    WAP_HIP(c, hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    c->workers.push_back(s);
+  }
+  while ((int)c->worker_events.size() < k + 1) {
+    hipEvent_t e;
+    WAP_HIP(c, hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    c->worker_events.push_back(e);
   }
   return WAP_OK;
 }
@@ -647,6 +667,11 @@ int wap_set_tuning(WapContext* c, const char* key, int value) {
     c->flow_storage = value;
     return WAP_OK;
   }
+  if (strcmp(key, "flow_streams") == 0) {
+    if (value < 1 || value > 8) return fail(c, WAP_E_ARG, "flow_streams must be 1..8");
+    c->flow_streams = value;
+    return WAP_OK;
+  }
   if (strcmp(key, "project_mapping") == 0) {
     if (value != -1 && value != 0 && value != 2 && value != 4 && value != 8 && value != 16)
       return fail(c, WAP_E_ARG, "project_mapping must be -1, 0, 2, 4, 8 or 16");
@@ -674,6 +699,10 @@ int wap_get_tuning(const WapContext* c, const char* key, int* value) {
   }
   if (strcmp(key, "flow_storage") == 0) {
     *value = c->flow_storage;
+    return WAP_OK;
+  }
+  if (strcmp(key, "flow_streams") == 0) {
+    *value = c->flow_streams;
     return WAP_OK;
   }
   if (strcmp(key, "project_mapping") == 0) {
@@ -722,6 +751,8 @@ int wap_ctx_destroy(WapContext* c) {
     if (t.frac) (void)hipFree(t.frac);
   }
   if (c->views) (void)hipFree(c->views);
+  for (hipStream_t w : c->workers) (void)hipStreamDestroy(w);
+  for (hipEvent_t e : c->worker_events) (void)hipEventDestroy(e);
   delete c;
   return WAP_OK;
 }
@@ -915,14 +946,45 @@ int wap_iteration(WapContext* c, const WapLoopConfig* L, float* vols, float* flo
   for (int f = 0; f < L->nstates; ++f)
     WAP_TRY(sart_prox_impl(c, L, vols + f * n, sino, ws[f], we[f], mask, st));
   if (L->do_flow) {
-    float* out;
-    WAP_TRY(scratch(c, "iter_flow", 3 * n, &out));
-    for (int f = 0; f + 1 < L->nstates; ++f) {
-      WAP_TRY(flow_multiscale_impl(c, L->dims, vols + f * n, vols + (f + 1) * n,
-                                   flows + f * 3 * n, L->flow_tol, L->flow_lambda,
-                                   L->flow_iters, L->flow_warps, L->flow_eta, L->flow_sigma,
-                                   L->flow_scales, L->cut, out, st));
-      WAP_HIP(c, wapk_copy(flows + f * 3 * n, out, 3 * n, st));
+    // the pairs' flows are independent: they run on up to flow_streams worker streams,
+    // each with its own scratch, so one pair's small pyramid levels overlap another's
+    // full-size level
+    const int K = std::max(1, std::min(c->flow_streams, L->nstates - 1));
+    if (K == 1) {
+      float* out;
+      WAP_TRY(scratch(c, "iter_flow", 3 * n, &out));
+      for (int f = 0; f + 1 < L->nstates; ++f) {
+        WAP_TRY(flow_multiscale_impl(c, L->dims, vols + f * n, vols + (f + 1) * n,
+                                     flows + f * 3 * n, L->flow_tol, L->flow_lambda,
+                                     L->flow_iters, L->flow_warps, L->flow_eta, L->flow_sigma,
+                                     L->flow_scales, L->cut, out, st));
+        WAP_HIP(c, wapk_copy(flows + f * 3 * n, out, 3 * n, st));
+      }
+    } else {
+      WAP_TRY(ensure_workers(c, K));
+      WAP_HIP(c, hipEventRecord(c->worker_events[K], st));
+      for (int k = 0; k < K; ++k)
+        WAP_HIP(c, hipStreamWaitEvent(c->workers[k], c->worker_events[K], 0));
+      int rc = WAP_OK;
+      for (int f = 0; f + 1 < L->nstates && rc == WAP_OK; ++f) {
+        hipStream_t ws_ = c->workers[f % K];
+        c->lane = 1 + f % K;
+        float* out;
+        rc = scratch(c, "iter_flow", 3 * n, &out);
+        if (rc == WAP_OK)
+          rc = flow_multiscale_impl(c, L->dims, vols + f * n, vols + (f + 1) * n,
+                                    flows + f * 3 * n, L->flow_tol, L->flow_lambda,
+                                    L->flow_iters, L->flow_warps, L->flow_eta, L->flow_sigma,
+                                    L->flow_scales, L->cut, out, ws_);
+        if (rc == WAP_OK && wapk_copy(flows + f * 3 * n, out, 3 * n, ws_) != hipSuccess)
+          rc = fail(c, WAP_E_HIP, "wap_iteration: flow copy failed");
+      }
+      c->lane = 0;
+      for (int k = 0; k < K; ++k) {
+        WAP_HIP(c, hipEventRecord(c->worker_events[k], c->workers[k]));
+        WAP_HIP(c, hipStreamWaitEvent(st, c->worker_events[k], 0));
+      }
+      if (rc != WAP_OK) return rc;
     }
   }
   if (L->do_warp) {
