@@ -79,6 +79,14 @@ int wap_frame_windows(int nframes, int total_views, float base, float* ftime, in
 /* Host: ROI mask imROI from m_roi[0..7] (Project/WaP_Tomo.cpp:391-417). */
 int wap_roi_mask(const int* dims, const int* roi8, float* host_mask);
 
+/* Host: nproj scan angles in degrees from the base-`base` van der Corput sequence spread
+ * over nseg equal circle segments (each sequence value v gives v*360/nseg + k*360/nseg,
+ * k = 0..nseg-1), with a 0-degree reference view inserted whenever the count reaches a
+ * multiple of zero_every (0 = never). An alternative to golden-ratio angles.
+ * Replaces Sampling_Model/Scanner_script/Low_discrepency_squence.m:15-44
+ * (Low_discrepency_squence(Np = nproj, Ns = nseg, p = base, Nz = zero_every)). */
+int wap_low_discrepancy_angles(int nproj, int nseg, int base, int zero_every, double* angles);
+
 /* Upload the view table used by the projector entry points (copied; synchronous). */
 int wap_upload_views(WapContext* ctx, const WapView* host_views, int nviews);
 

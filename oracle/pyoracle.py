@@ -86,6 +86,7 @@ class Oracle:
         L.orc_view_order.argtypes = [C.c_int, C.c_int, i32p]
         L.orc_frame_windows.argtypes = [C.c_int, C.c_int, C.c_float, f32p, i32p, i32p]
         L.orc_roi_mask.argtypes = [i32p, i32p, f32p]
+        L.orc_low_discrepancy.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, f64p]
         L.orc_forward_project_f32.argtypes = [f32p, i32p, C.c_float, vp, C.c_int, C.c_int,
                                               f32p, f32p, f32p]
         L.orc_forward_project_f64.argtypes = [f64p, i32p, C.c_double, vp, C.c_int, C.c_int,
@@ -154,6 +155,12 @@ class Oracle:
         m = np.zeros(shape, np.float32)
         self.lib.orc_roi_mask(_dims(shape), _c(roi8, np.int32), m)
         return m
+
+    def low_discrepancy(self, np_, ns, p, nz):
+        out = np.zeros(np_, np.float64)
+        n = self.lib.orc_low_discrepancy(np_, ns, p, nz, out)
+        assert n == np_
+        return out
 
     # -- projector ------------------------------------------------------------------
     def forward_project(self, vol, view, nu, nv, sd, f64=False):

@@ -1188,6 +1188,28 @@ void orc_frame_windows(int nframes, int total_views, float base, float* ftime, i
   }
 }
 
+// Scan angles (degrees) from a base-p van der Corput sequence over ns circle segments,
+// with a 0-degree reference view inserted whenever the count reaches a multiple of nz
+// (Sampling_Model/Scanner_script/Low_discrepency_squence.m:15-44; nz = 0 never inserts,
+// as MATLAB's mod(n, 0) = n). Written as the script's double loop, term by term.
+int orc_low_discrepancy(int np, int ns, int p, int nz, double* seq) {
+  int len = 0;
+  for (long long i = 0; len < np; ++i) {
+    if (len > 0 && nz > 0 && len % nz == 0) seq[len++] = 0.0;
+    long long b = i;
+    double r = 0.0, q = 1.0 / p;
+    while (b != 0) {
+      const long long a = b % p;
+      r = r + (double)a * q;
+      q = q / p;
+      b = b / p;
+    }
+    r = 360.0 * r / ns;
+    for (int k = 0; len < np && k < ns; ++k) seq[len++] = r + (double)k * 360.0 / ns;
+  }
+  return len;
+}
+
 // imROI, Project/WaP_Tomo.cpp:391-417, from explicit m_roi[0..7]
 void orc_roi_mask(const int* dims, const int* r, float* mask) {
   const int W = dims[0], H = dims[1], D = dims[2];

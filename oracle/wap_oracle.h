@@ -25,6 +25,7 @@ void orc_view_order(int first, int last, int* order);
 void orc_frame_windows(int nframes, int total_views, float base, float* ftime, int* start,
                        int* end);
 void orc_roi_mask(const int* dims, const int* roi8, float* mask);
+int orc_low_discrepancy(int np, int ns, int p, int nz, double* seq);
 
 /* projector: proj/raylen/weight are Nu*Nv per view */
 void orc_forward_project_f32(const float* vol, const int* dims, float sd, const WapView* v,

@@ -112,6 +112,24 @@ def test_frame_windows_equal_oracle(oracle, nf, total):
     np.testing.assert_array_equal(np.array(e[:]), oe)
 
 
+@pytest.mark.parametrize("np_,ns,p,nz", [(0, 1, 2, 0), (1, 1, 2, 0), (100, 1, 2, 50),
+                                         (4000, 10, 2, 10000), (333, 7, 3, 20), (64, 4, 5, 1),
+                                         (1000, 3, 7, 0)])
+def test_low_discrepancy_angles_equal_oracle(oracle, np_, ns, p, nz):
+    got = wap_amd.low_discrepancy_angles(np_, ns, p, nz)
+    want = oracle.low_discrepancy(np_, ns, p, nz)
+    assert np.array(got, np.float64).tobytes() == want.tobytes()
+
+
+def test_low_discrepancy_angles_reject_bad_arguments():
+    lib = wap_amd.load()
+    out = (C.c_double * 4)()
+    for args in [(-1, 1, 2, 0), (4, 0, 2, 0), (4, 1, 1, 0), (4, 1, 2, -1)]:
+        assert lib.wap_low_discrepancy_angles(*args, out) == -1
+    assert lib.wap_low_discrepancy_angles(4, 1, 2, 0, None) == -1
+    assert lib.wap_low_discrepancy_angles(0, 1, 2, 0, None) == 0
+
+
 @pytest.mark.parametrize("roi", [(0,) * 8, (2,) * 8, (1, 3, 2, 0, 4, 1, 5, 2), (8,) * 8])
 def test_roi_mask_equals_oracle(oracle, roi):
     lib = wap_amd.load()

@@ -275,6 +275,60 @@ def test_frame_windows(oracle):
     np.testing.assert_array_equal(e, [3, 6, 9])
 
 
+def _vdc(i, p):
+    """Exact radical inverse of i in base p (most significant digit last)."""
+    from fractions import Fraction
+    r, q = Fraction(0), Fraction(1, p)
+    while i:
+        r += (i % p) * q
+        q /= p
+        i //= p
+    return r
+
+
+def test_low_discrepancy_closed_forms(oracle):
+    from fractions import Fraction
+    # base 2, one segment: 360 * (0, 1/2, 1/4, 3/4, 1/8, 5/8, 3/8, 7/8), exact in double
+    assert list(oracle.low_discrepancy(8, 1, 2, 0)) == [0, 180, 90, 270, 45, 225, 135, 315]
+    # base 3: 0, 1/3, 2/3, 1/9, 4/9, 7/9, 2/9, 5/9, 8/9 of the circle
+    np.testing.assert_allclose(oracle.low_discrepancy(9, 1, 3, 0),
+                               [0, 120, 240, 40, 160, 280, 80, 200, 320], rtol=0, atol=1e-12)
+    # 2^m base-2 points are exactly the 2^m equispaced angles
+    a = oracle.low_discrepancy(256, 1, 2, 0)
+    assert sorted(a) == [360.0 * j / 256 for j in range(256)]
+    # segments (the scanner script's Np = 4000, Ns = 10): row i of the Ns-column
+    # reshape is 36 * vdc(i) + 36 k, k = 0..9; every segment holds Np / Ns angles
+    for base in (2, 3, 5):
+        a = oracle.low_discrepancy(4000, 10, base, 10000)
+        want = [float(Fraction(360) * _vdc(i, base) / 10 + Fraction(360 * k, 10))
+                for i in range(400) for k in range(10)]
+        np.testing.assert_allclose(a, want, rtol=1e-14, atol=1e-12)
+        counts = np.bincount((a // 36.0).astype(int), minlength=10)
+        assert list(counts) == [400] * 10
+
+
+def test_low_discrepancy_reference_views(oracle):
+    # the demo config (Np = 100, Ns = 1, p = 2, Nz = 50): one 0-degree view at index 50
+    a = oracle.low_discrepancy(100, 1, 2, 50)
+    assert a[50] == 0.0
+    rest = np.delete(a, 50)
+    assert list(rest) == [float(360 * _vdc(i, 2)) for i in range(99)]
+    # with one segment the inserted views land at every multiple of Nz
+    a = oracle.low_discrepancy(400, 1, 2, 50)
+    zeros = [int(i) for i in np.flatnonzero(a == 0.0)]
+    assert zeros == [0, 50, 100, 150, 200, 250, 300, 350]
+    # Nz = 1: every other view is the reference view
+    a = oracle.low_discrepancy(9, 1, 2, 1)
+    assert list(a) == [0, 0, 180, 0, 90, 0, 270, 0, 45]
+    # with Ns segments the check runs between batches only: with Ns = 4, Nz = 6 the
+    # count is 4, 8, 12 (insert -> 13), 17, 21, 25, 29 -- one insertion, at index 12
+    a = oracle.low_discrepancy(30, 4, 2, 6)
+    assert list(a[8:12]) == [22.5, 112.5, 202.5, 292.5]
+    assert a[12] == 0.0 and list(a[13:17]) == [67.5, 157.5, 247.5, 337.5]
+    assert a[29] == 78.75
+    assert list(np.flatnonzero(a == 0.0)) == [0, 12]
+
+
 def test_roi_mask_brute_force(oracle):
     shape = (13, 11, 15)
     r = [2, 3, 1, 2, 3, 2, 4, 3]

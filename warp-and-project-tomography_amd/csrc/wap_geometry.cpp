@@ -117,6 +117,28 @@ extern "C" int wap_frame_windows(int nframes, int total_views, float base, float
   return WAP_OK;
 }
 
+extern "C" int wap_low_discrepancy_angles(int nproj, int nseg, int base, int zero_every,
+                                          double* angles) {
+  if (nproj < 0 || nseg < 1 || base < 2 || zero_every < 0 || (nproj > 0 && !angles))
+    return WAP_E_ARG;
+  const double step = 360.0;
+  int n = 0;
+  for (long long idx = 0; n < nproj; ++idx) {
+    // reference view at 0 degrees each time the count reaches a multiple of zero_every
+    if (zero_every > 0 && n > 0 && n % zero_every == 0) angles[n++] = 0.0;
+    // radical inverse of idx in the given base, digits taken least significant first and
+    // summed in that order (the order fixes the double result)
+    double frac = 0.0, weight = 1.0 / base;
+    for (long long rest = idx; rest != 0; rest /= base) {
+      frac = frac + (double)(rest % base) * weight;
+      weight = weight / base;
+    }
+    const double first = step * frac / nseg;
+    for (int seg = 0; seg < nseg && n < nproj; ++seg) angles[n++] = first + (double)seg * step / nseg;
+  }
+  return WAP_OK;
+}
+
 extern "C" int wap_roi_mask(const int* dims, const int* r, float* mask) {
   if (!dims || !r || !mask) return WAP_E_ARG;
   const int W = dims[0], H = dims[1], D = dims[2];

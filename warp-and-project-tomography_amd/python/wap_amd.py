@@ -57,6 +57,8 @@ SIGNATURES = {
     "wap_view_order": [C.c_int, C.c_int, _ip],
     "wap_frame_windows": [C.c_int, C.c_int, C.c_float, _fp, _ip, _ip],
     "wap_roi_mask": [_ip, _ip, _fp],
+    "wap_low_discrepancy_angles": [C.c_int, C.c_int, C.c_int, C.c_int,
+                                   C.POINTER(C.c_double)],
     "wap_upload_views": [_vp, C.POINTER(WapView), C.c_int],
     "wap_forward_project": [_vp, _vp, C.c_size_t, _ip, C.c_float, C.c_int, C.c_int, C.c_int,
                             C.c_int, _vp, _vp, _vp, _vp],
@@ -142,6 +144,17 @@ def views(geom, angles):
         if r != WAP_OK:
             raise ValueError(f"wap_view_geometry failed ({r})")
     return arr
+
+
+def low_discrepancy_angles(nproj, nseg=1, base=2, zero_every=0):
+    """Scan angles in degrees (wap_low_discrepancy_angles): base-`base` van der Corput
+    values spread over `nseg` circle segments, a 0-degree view every `zero_every`."""
+    lib = load()
+    out = (C.c_double * max(int(nproj), 1))()
+    r = lib.wap_low_discrepancy_angles(int(nproj), int(nseg), int(base), int(zero_every), out)
+    if r != WAP_OK:
+        raise ValueError(f"wap_low_discrepancy_angles failed ({r})")
+    return list(out[:int(nproj)])
 
 
 def _dims(t):
