@@ -27,6 +27,8 @@ extern "C" {
 #define WAP_E_HIP (-2)     /* a HIP runtime call failed */
 #define WAP_E_NOMEM (-3)   /* device allocation failed */
 #define WAP_E_STATE (-4)   /* missing prerequisite (e.g. views not uploaded) */
+#define WAP_E_IO (-5)      /* a file could not be read or written */
+#define WAP_CLI_EXIT 1     /* wap_cli_parse: help, version or no arguments; print msg, exit 0 */
 
 typedef struct WapContext WapContext;
 
@@ -86,6 +88,40 @@ int wap_roi_mask(const int* dims, const int* roi8, float* host_mask);
  * Replaces Sampling_Model/Scanner_script/Low_discrepency_squence.m:15-44
  * (Low_discrepency_squence(Np = nproj, Ns = nseg, p = base, Nz = zero_every)). */
 int wap_low_discrepancy_angles(int nproj, int nseg, int base, int zero_every, double* angles);
+
+/* Host: text of the last failing host-side helper below (file formats, command line) on
+ * the calling thread. */
+const char* wap_host_error(void);
+
+/* Host: read an image or projection stack as float, planar x-fastest, with
+ * dims4 = W, H, D, C. Format by extension: .hdr/.img (Analyze 7.5, either byte order,
+ * datatypes u8/i16/i32/f32/f64, header scale factor) and .nii (single file, vox_offset);
+ * .tif/.tiff (uncompressed stripped pages -> D, samples per pixel -> C, 8..64-bit
+ * unsigned/signed/float); .mha/.mhd (MetaImage, uncompressed); anything else is raw
+ * little-endian float32 whose dims4 the caller passes in (C = 0 means 1). With
+ * out == NULL only dims4 is filled; capacity is the float count at out.
+ * Replaces the projection-stack load image(pInput[0]).get_split('z', projNo)
+ * (Project/WaP_Tomo.cpp:339; CImg::load_analyze, Project/CImg.h:47829). */
+int wap_read_image(const char* path, int* dims4, float* out, size_t capacity);
+
+/* Host: write a float W x H x D x C image as Analyze 7.5 (path .hdr/.img or no extension
+ * -> .hdr + .img pair; .nii -> one file), header fields filled as CImg::save_analyze
+ * fills them (Project/CImg.h:51715; saves at Project/WaP_Tomo.cpp:766-771,1418,1669).
+ * voxel_size may be NULL (1, 1, 1). */
+int wap_write_analyze(const char* path, const int* dims4, const float* data,
+                      const float* voxel_size);
+
+/* Host: read up to n whitespace-separated angles (degrees) from a text file; unread
+ * entries are 0, *nread (may be NULL) gets the count read. Replaces
+ * Project/WaP_Tomo.cpp:324-331 (ours_foam_compress.txt). */
+int wap_read_angles(const char* path, int n, double* out, int* nread);
+
+/* Host: parse the reconstruction driver's command line into *cfg (defaults first).
+ * Returns WAP_OK, WAP_CLI_EXIT (help/version/no arguments: msg holds the text to print)
+ * or WAP_E_ARG (msg holds the errors). Replaces the argtable2 table and parse of
+ * Project/main.cpp:15-176. */
+int wap_cli_parse(int argc, const char* const* argv, WapCliConfig* cfg, char* msg,
+                  size_t msglen);
 
 /* Upload the view table used by the projector entry points (copied; synchronous). */
 int wap_upload_views(WapContext* ctx, const WapView* host_views, int nviews);

@@ -68,6 +68,44 @@ typedef struct WapLoopConfig {
   float timeunit;     /* views per state, for the warp time offset */
 } WapLoopConfig;
 
+/* Settings of the reconstruction driver's command line (Project/main.cpp:15-143). The
+ * reference parses them with argtable2. Flags and defaults are the same. Options marked
+ * "required" have a minimum count of 1 in the reference's table. Fields the loop does not
+ * read are parsed and kept, as there. */
+#define WAP_CLI_PATH 1024
+#define WAP_CLI_MAXFILES 20
+typedef struct WapCliConfig {
+  double sigma;          /* -s/--Sigma: TV weight TV_w (0.3) */
+  double tau;            /* -t/--Tau: temporal-prior weight TP_w (0.3) */
+  double lambda;         /* -l/--Lambda (0.1; not read by the loop) */
+  double alpha;          /* -a/--Alpha: SART relaxation (0.1) */
+  double voxel_spacing;  /* -c/--voxelspacing (1.0) */
+  double det_spacing;    /* -d (1.0) */
+  double sid;            /* -g/--sid: source to rotation centre (400) */
+  double sdd;            /* -j/--sdd: source to detector (200) */
+  double oxyz[3];        /* --oxyz: volume centre offset, required, 1..3 values (rest 0) */
+  double start_deg[WAP_CLI_MAXFILES]; /* --sdg: required, 1..20 values (InitViews only) */
+  int n_start_deg;
+  int vol[3];            /* -f/--XYZ: volume W H D, required, 1..3 values (rest 63) */
+  int img[2];            /* -u/--imgUV: detector W H, required, 1..2 values (rest 512) */
+  int nframes;           /* -v/--nframes (1) */
+  int nrounds;           /* -r (1) */
+  int algo_iters;        /* -b/--AlgorithmIter (2; not read by the loop) */
+  int sart_iters;        /* -p/--SartIter (2; not read by the loop) */
+  int total_proj;        /* projections: 192, fixed in the reference driver; --nproj */
+  int n_proj_files;      /* -i/--projsvolume: required, 1..20 files; the first is read */
+  char proj_files[WAP_CLI_MAXFILES][WAP_CLI_PATH];
+  char output[WAP_CLI_PATH]; /* -o: output prefix ("Test/test") */
+  char prior[16];        /* --prior {STV,SAD,ATV} (ATV; not read by the loop) */
+  char bp[16];           /* --bp {Voxelbased,Raybased} (Voxelbased) */
+  /* additions, no reference flag */
+  char angles[WAP_CLI_PATH]; /* --angles: angle list ("ours_foam_compress.txt") */
+  int lds[3];            /* --lds Ns,p,Nz: low-discrepancy angles instead of the list */
+  int use_lds;
+  int save;              /* --save {none,final,all} (final) */
+  int device;            /* --device (0) */
+} WapCliConfig;
+
 /* Storage element type of volumes, flows and sinograms. */
 typedef enum WapDType { WAP_F32 = 0, WAP_BF16 = 1, WAP_F16 = 2 } WapDType;
 

@@ -4,7 +4,9 @@
 #include <hip/hip_runtime_api.h>
 #include <math.h>
 #include <stdio.h>
+#include <stdarg.h>
 #include <string.h>
+#include <sys/stat.h>
 
 #include <algorithm>
 #include <fstream>
@@ -627,14 +629,60 @@ PSART::PSART(double d1, double d2, double d3, const char* outputfile, int* imgwh
     std::ifstream in("ours_foam_compress.txt");
     for (int i = 0; in && i < projNo; ++i) in >> m_angles[i];
   }
-  Ok(wap_ctx_create(&m_ctx, 0));
+  (void)hipSetDevice(DefaultDevice);
+  Ok(wap_ctx_create(&m_ctx, DefaultDevice));
   m_sino.assign(imgwh[0], imgwh[1], projNo, 1, 0.0f);
   if (projsfile && projsfile[0]) {
-    // raw float32 stack of projNo images of W x H (the reference loads a TIFF stack)
-    std::ifstream f(projsfile[0], std::ios::binary);
-    std::vector<float> h(m_sino.size());
-    if (f.read((char*)h.data(), h.size() * sizeof(float))) SetProjections(h.data(), projNo);
+    // the stack holds one W x H image per view along z (Project/WaP_Tomo.cpp:339); a
+    // raw float32 file is taken as projNo x H x W
+    int dims[4] = {imgwh[0], imgwh[1], projNo, 1};
+    int r = wap_read_image(projsfile[0], dims, nullptr, 0);
+    if (r == WAP_OK && (dims[0] != imgwh[0] || dims[1] != imgwh[1] || dims[2] != projNo ||
+                        dims[3] != 1)) {
+      fprintf(stderr, "wap: %s holds %d x %d x %d x %d, expected %d x %d x %d x 1\n",
+              projsfile[0], dims[0], dims[1], dims[2], dims[3], imgwh[0], imgwh[1], projNo);
+      m_err = WAP_E_ARG;
+    } else if (r != WAP_OK) {
+      fprintf(stderr, "wap: %s\n", wap_host_error());
+      m_err = r;
+    } else {
+      std::vector<float> h(m_sino.size());
+      r = wap_read_image(projsfile[0], dims, h.data(), h.size());
+      if (r == WAP_OK) SetProjections(h.data(), projNo);
+      else fprintf(stderr, "wap: %s\n", wap_host_error()), m_err = r;
+    }
   }
+}
+
+int PSART::DefaultDevice = 0;
+
+// Downloads one channel (or all, channel < 0) of a device image and writes it as an
+// Analyze pair, creating missing parent directories.
+void PSART::Save(const image& img, int channel, const std::string& path) {
+  if (img.is_empty()) return;
+  std::vector<float> h(img.size());
+  img.download(h.data());
+  int dims[4] = {img.width(), img.height(), img.depth(), channel < 0 ? img.spectrum() : 1};
+  const float* src = h.data();
+  if (channel >= 0) src += (size_t)channel * img.width() * img.height() * img.depth();
+  for (size_t k = path.find('/'); k != std::string::npos; k = path.find('/', k + 1))
+    if (k > 0) mkdir(path.substr(0, k).c_str(), 0755);
+  // the reference saves without voxel sizes (pixdim 1, 1, 1)
+  const int r = wap_write_analyze(path.c_str(), dims, src, nullptr);
+  if (r != WAP_OK) {
+    fprintf(stderr, "wap: %s\n", wap_host_error());
+    m_err = r;
+  }
+}
+
+static std::string fmt(const char* f, ...) __attribute__((format(printf, 1, 2)));
+static std::string fmt(const char* f, ...) {
+  char buf[2048];
+  va_list ap;
+  va_start(ap, f);
+  vsnprintf(buf, sizeof(buf), f, ap);
+  va_end(ap);
+  return buf;
 }
 
 PSART::~PSART() {
@@ -754,28 +802,30 @@ WapLoopConfig PSART::LoopConfig() const {
 }
 
 void PSART::SART_Proximal_Operator(int iframes, int nb_sart, int outloop, int stIter, int flevel) {
-  (void)outloop;
-  (void)stIter;
   Frame_item& f = m_Data->m_Frameslist[flevel][iframes];
   WapLoopConfig c = LoopConfig();
   c.nb_sart = nb_sart;
   c.nstates = (int)m_Data->m_Frameslist[flevel].size();
   Ok(wap_sart_prox(m_ctx, &c, f.fVolume.data(), m_sino.data(), f.fStartproj, f.fEndproj,
                    m_roiMask.data(), nullptr));
+  if (save_mode == 2)
+    Save(f.fVolume, -1, fmt("%s.f%03d.out%02d.st%02d.flevel%02d.hdr", m_output.c_str(), iframes,
+                            outloop, stIter, flevel));
 }
 
 void PSART::WarpAndProjct(int iproj, int iframes, int* dimension, float rtime, float* aabb,
                           int outloop, int warpiter, char* _file_header, int flevel) {
   (void)dimension;
   (void)aabb;
-  (void)outloop;
-  (void)warpiter;
-  (void)_file_header;
   WapLoopConfig c = LoopConfig();
   c.nstates = (int)m_Data->m_Frameslist[flevel].size();
   Ok(wap_warp_and_project(m_ctx, &c, m_Data->m_LevelVolumes[flevel].data(),
                           m_Data->m_LevelFlows[flevel].data(), m_sino.data(), iframes, iproj,
                           rtime, m_roiMask.data(), nullptr));
+  if (save_mode == 2 && warpiter == 0)
+    Save(m_Data->m_Frameslist[flevel][iframes].fVolume, -1,
+         fmt("%s/f%03d.p%03d.warp%03d.out%02d.flevel%02d.hdr",
+             _file_header ? _file_header : "WarpStep", iframes, iproj, warpiter, outloop, flevel));
 }
 
 // The outer loop (Project/WaP_Tomo.cpp:473-877) on the device: per temporal level, the
@@ -819,6 +869,23 @@ void PSART::Render() {
                              c.flow_scales, c.cut, out.data(), nullptr));
       a.fFlow = out;
       m_flowSet[level * 131 + f] = 1;
+      if (save_mode == 2) {
+        // the pair warped onto each other and the flow channels (Project/WaP_Tomo.cpp:760-771)
+        const float smoothness = c.flow_lambda, huber = 1.0f;
+        image w1(m_voldims[0], m_voldims[1], m_voldims[2], 1), w2 = w1;
+        Ok(wap_advect(m_ctx, m_voldims, aabb, a.fFlow.data(), b.fVolume.data(), 1, -1.0f,
+                      w1.data(), nullptr));
+        Ok(wap_advect(m_ctx, m_voldims, aabb, a.fFlow.data(), a.fVolume.data(), 1, +1.0f,
+                      w2.data(), nullptr));
+        Save(w1, -1, fmt("warpedVolume/warpedI%03d.s%1.3f.h%3.3f.f%03d.outloop%03d.f%03d_back.hdr",
+                         f, smoothness, huber, f, 0, f));
+        Save(w2, -1, fmt("warpedVolume/warpedI%03d.s%1.3f.h%3.3f.f%03d.outloop%03d.f%03d_forward.hdr",
+                         f + 1, smoothness, huber, f, 0, f));
+        const char* comp[3] = {"u", "v", "w"};
+        for (int ch = 0; ch < 3; ++ch)
+          Save(a.fFlow, ch, fmt("VelocityField/Flow%03d.s%2.3f.h%1.3f.f%03d.outloop%03d.%s.hdr",
+                                m_voldims[0], smoothness, huber, f, 0, comp[ch]));
+      }
     }
     const float trange = m_Data->m_tUnitInLevel[level];
     for (int fidx = 0; fidx < m_Data->m_nframes; ++fidx) {
@@ -827,7 +894,8 @@ void PSART::Render() {
         float td = (ik - fr.fTime) / trange;
         td = std::min(std::max(-1.0f, td), 1.0f);
         int dims[3] = {m_voldims[0], m_voldims[1], m_voldims[2]};
-        WarpAndProjct(ik, fidx, dims, td, const_cast<float*>(aabb), 0, 0, nullptr, level);
+        char base[] = "WarpStep";
+        WarpAndProjct(ik, fidx, dims, td, const_cast<float*>(aabb), 0, 0, base, level);
       }
     }
     if (level < nlevels - 1) {
@@ -839,6 +907,12 @@ void PSART::Render() {
         for (int k = 2 * i; k <= 2 * i + 1; ++k)
           if (k < (int)fine.size()) fine[k].fVolume = m_Data->m_Frameslist[level][i].fVolume;
     }
+  }
+  if (save_mode == 1) {
+    const int last = nlevels - 1;
+    for (int f = 0; f < m_Data->m_fInLevel[last]; ++f)
+      Save(m_Data->m_Frameslist[last][f].fVolume, -1,
+           fmt("%s.f%03d.flevel%02d.final.hdr", m_output.c_str(), f, last));
   }
   (void)hipDeviceSynchronize();
 }

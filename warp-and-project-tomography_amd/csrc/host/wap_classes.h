@@ -10,8 +10,9 @@
 //
 // Differences, by design: images live on the device (DeviceImage instead of CImg);
 // the projector state is per object instead of file-static; there is no GL window
-// (Render() is the headless driver) and no TIFF/Analyze I/O (projections and angles
-// come from raw float32 / text files or from memory through SetProjections/SetAngles).
+// (Render() is the headless driver). Projections are read by wap_read_image (Analyze,
+// NIfTI, TIFF, MetaImage or raw float32) or given from memory (SetProjections), angles
+// from the text list or SetAngles; volumes are written as Analyze by save_mode.
 #pragma once
 
 #include <stddef.h>
@@ -261,6 +262,15 @@ class PSART : public Scene {
   WapContext* Context() { return m_ctx; }
   const WapView* Views() const { return m_views.data(); }
   int LastError() const { return m_err; }
+  // Files written (the reference always writes its step files):
+  //   0 none (default); 1 the final volume of every state of the last level,
+  //   "<output>.f%03d.flevel%02d.final.hdr"; 2 the reference's files: each SART-prox
+  //   result "<output>.f%03d.out%02d.st%02d.flevel%02d.hdr" (Project/WaP_Tomo.cpp:1669),
+  //   warped pair volumes in warpedVolume/ and flow channels in VelocityField/
+  //   (:766-771), each warp step in WarpStep/ (:1418).
+  int save_mode = 0;
+  // device of the next PSART's context (the reference has one GPU-less process)
+  static int DefaultDevice;
   // outer-loop schedule (reference Render locals, Project/WaP_Tomo.cpp:478-492, 711-727)
   int nb_volOpt[2] = {2, 10};
   int nb_proxiSART = 2;
@@ -271,6 +281,7 @@ class PSART : public Scene {
 
  private:
   void BuildViews();
+  void Save(const image& img, int channel, const std::string& path);
   WapLoopConfig LoopConfig() const;
   bool Ok(int r);
 

@@ -40,6 +40,20 @@ class WapLoopConfig(C.Structure):
                 ("do_warp", C.c_int), ("timeunit", C.c_float)]
 
 
+class WapCliConfig(C.Structure):
+    _fields_ = [("sigma", C.c_double), ("tau", C.c_double), ("lambda_", C.c_double),
+                ("alpha", C.c_double), ("voxel_spacing", C.c_double),
+                ("det_spacing", C.c_double), ("sid", C.c_double), ("sdd", C.c_double),
+                ("oxyz", C.c_double * 3), ("start_deg", C.c_double * 20),
+                ("n_start_deg", C.c_int), ("vol", C.c_int * 3), ("img", C.c_int * 2),
+                ("nframes", C.c_int), ("nrounds", C.c_int), ("algo_iters", C.c_int),
+                ("sart_iters", C.c_int), ("total_proj", C.c_int), ("n_proj_files", C.c_int),
+                ("proj_files", (C.c_char * 1024) * 20), ("output", C.c_char * 1024),
+                ("prior", C.c_char * 16), ("bp", C.c_char * 16), ("angles", C.c_char * 1024),
+                ("lds", C.c_int * 3), ("use_lds", C.c_int), ("save", C.c_int),
+                ("device", C.c_int)]
+
+
 assert C.sizeof(WapScanGeometry) == 64 and C.sizeof(WapView) == 256
 
 # name -> argtypes (restype int unless noted)
@@ -59,6 +73,12 @@ SIGNATURES = {
     "wap_roi_mask": [_ip, _ip, _fp],
     "wap_low_discrepancy_angles": [C.c_int, C.c_int, C.c_int, C.c_int,
                                    C.POINTER(C.c_double)],
+    "wap_host_error": [],
+    "wap_read_image": [C.c_char_p, _ip, _fp, C.c_size_t],
+    "wap_write_analyze": [C.c_char_p, _ip, _fp, _fp],
+    "wap_read_angles": [C.c_char_p, C.c_int, C.POINTER(C.c_double), _ip],
+    "wap_cli_parse": [C.c_int, C.POINTER(C.c_char_p), C.POINTER(WapCliConfig), C.c_char_p,
+                      C.c_size_t],
     "wap_upload_views": [_vp, C.POINTER(WapView), C.c_int],
     "wap_forward_project": [_vp, _vp, C.c_size_t, _ip, C.c_float, C.c_int, C.c_int, C.c_int,
                             C.c_int, _vp, _vp, _vp, _vp],
@@ -100,7 +120,7 @@ def load(path=LIB_PATH):
     for name, args in SIGNATURES.items():
         fn = getattr(lib, name)
         fn.argtypes = args
-        fn.restype = C.c_char_p if name == "wap_last_error" else C.c_int
+        fn.restype = C.c_char_p if name in ("wap_last_error", "wap_host_error") else C.c_int
     _lib = lib
     return lib
 
@@ -155,6 +175,63 @@ def low_discrepancy_angles(nproj, nseg=1, base=2, zero_every=0):
     if r != WAP_OK:
         raise ValueError(f"wap_low_discrepancy_angles failed ({r})")
     return list(out[:int(nproj)])
+
+
+WAP_CLI_EXIT = 1
+
+
+def host_error():
+    return load().wap_host_error().decode()
+
+
+def read_image(path, dims=None):
+    """Read an image or projection stack (wap_read_image) as a float32 numpy array of
+    shape (C, D, H, W). `dims` (W, H, D[, C]) is needed for raw float32 files only."""
+    import numpy as np
+    lib = load()
+    d = ints(*(list(dims or (0, 0, 0, 0)) + [0] * (4 - len(dims or ())))[:4])
+    r = lib.wap_read_image(os.fsencode(path), d, None, 0)
+    if r != WAP_OK:
+        raise OSError(f"wap_read_image({path}): {host_error()} ({r})")
+    w, h, dd, c = d[:]
+    out = np.empty((c, dd, h, w), np.float32)
+    r = lib.wap_read_image(os.fsencode(path), d, out.ctypes.data_as(_fp), out.size)
+    if r != WAP_OK:
+        raise OSError(f"wap_read_image({path}): {host_error()} ({r})")
+    return out
+
+
+def write_analyze(path, arr, voxel_size=None):
+    """Write a float32 array of shape (D, H, W) or (C, D, H, W) as Analyze 7.5."""
+    import numpy as np
+    a = np.ascontiguousarray(arr, np.float32)
+    if a.ndim == 3:
+        a = a[None]
+    c, d, h, w = a.shape
+    vs = floats(*voxel_size) if voxel_size is not None else None
+    r = load().wap_write_analyze(os.fsencode(path), ints(w, h, d, c), a.ctypes.data_as(_fp), vs)
+    if r != WAP_OK:
+        raise OSError(f"wap_write_analyze({path}): {host_error()} ({r})")
+
+
+def read_angles(path, n):
+    lib = load()
+    out = (C.c_double * max(n, 1))()
+    got = C.c_int()
+    r = lib.wap_read_angles(os.fsencode(path), int(n), out, C.byref(got))
+    if r != WAP_OK:
+        raise OSError(f"wap_read_angles({path}): {host_error()} ({r})")
+    return list(out[:n]), got.value
+
+
+def cli_parse(args, prog="wap_tomo"):
+    """Parse a driver command line (wap_cli_parse): returns (status, config, message)."""
+    argv = [prog] + list(args)
+    arr = (C.c_char_p * len(argv))(*[a.encode() for a in argv])
+    cfg = WapCliConfig()
+    msg = C.create_string_buffer(1 << 16)
+    r = load().wap_cli_parse(len(argv), arr, C.byref(cfg), msg, len(msg))
+    return r, cfg, msg.value.decode()
 
 
 def _dims(t):
