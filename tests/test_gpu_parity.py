@@ -269,6 +269,69 @@ def test_flow_iteration_variants_identical(wctx, oracle, variant):
         wctx.set_tuning("flow_variant", prev)
 
 
+def _flow_case(oracle, shape, roi, seed):
+    I1 = oracle.blur(helpers.blob_volume(shape, seed), 1.0)
+    I2 = oracle.advect(np.full((3,) + shape, 0.4, np.float32), I1, 1.0)
+    return I1, I2, [0, shape[2], 0, shape[1], 0, shape[0]]
+
+
+PAIR_CASES = [((20, 18, 70), [3, 2, 1, 2, 2, 3]), ((14, 16, 18), [2] * 6),
+              ((37, 11, 67), [1, 0, 2, 1, 0, 3]), ((70, 40, 37), [4, 1, 3, 0, 2, 5])]
+
+
+@pytest.mark.parametrize("tol", [0.008, 0.3, 0.9, 1e9])
+@pytest.mark.parametrize("shape_id", list(range(1, 11)))
+def test_flow_pair_form_identical(wctx, oracle, shape_id, tol):
+    """The two-iterations-per-pass form, each tile shape forced, gives the oracle's floats
+    (v, all ten duals and the iteration count), also when the residual stops a warp early
+    (tol 0.3 and 0.9 stop warps at iteration 50 or 100, between queued pairs) and when the
+    reference's initial err = 1 runs no iteration at all (tol 1e9)."""
+    prev = wctx.get_tuning("flow_pair")
+    wctx.set_tuning("flow_pair", shape_id)
+    try:
+        for shape, roi in PAIR_CASES:
+            I1, I2, aabb = _flow_case(oracle, shape, roi, 13)
+            z3 = np.zeros((3,) + shape, np.float32)
+            z1 = np.zeros(shape, np.float32)
+            V = torch.zeros((3,) + shape, device=DEV)
+            Y = torch.zeros((10,) + shape, device=DEV)
+            print(f"pair shape {shape_id} grid {shape} roi {roi} tol {tol}", flush=True)
+            it = wctx.flow_solve(T(I1), T(I2), V, Y, tol, 0.05, 120, 2, aabb, roi, count=True)
+            assert wctx.get_tuning("flow_pair_used") == shape_id
+            v, y1, y2, y3, y4, it_ref = oracle.flow_single(I1, I2, z3, z3, z3, z3, z1, tol, 0.05,
+                                                           120, 2, aabb, roi)
+            assert it == it_ref
+            np.testing.assert_array_equal(H(V), v)
+            np.testing.assert_array_equal(H(Y), np.concatenate([y1, y2, y3, y4[None]]))
+    finally:
+        wctx.set_tuning("flow_pair", prev)
+
+
+@pytest.mark.parametrize("storage", [1, 2])
+def test_flow_pair_form_half_storage(wctx, oracle, storage):
+    """With bfloat16/half storage of the duals the pair form equals the single-iteration
+    form bit for bit (it rounds the intermediate duals exactly as a store would)."""
+    prev = (wctx.get_tuning("flow_pair"), wctx.get_tuning("flow_storage"))
+    wctx.set_tuning("flow_storage", storage)
+    try:
+        shape, roi = PAIR_CASES[3]
+        I1, I2, aabb = _flow_case(oracle, shape, roi, 14)
+        outs = []
+        for pair in (0, 1, 3):
+            wctx.set_tuning("flow_pair", pair)
+            V = torch.zeros((3,) + shape, device=DEV)
+            Y = torch.zeros((10,) + shape, device=DEV)
+            wctx.flow_solve(T(I1), T(I2), V, Y, 0.008, 0.05, 60, 2, aabb, roi)
+            outs.append((H(V), H(Y)))
+        for v, y in outs[1:]:
+            np.testing.assert_array_equal(v, outs[0][0])
+            np.testing.assert_array_equal(y, outs[0][1])
+        assert np.abs(outs[0][0]).max() > 0.01
+    finally:
+        wctx.set_tuning("flow_pair", prev[0])
+        wctx.set_tuning("flow_storage", prev[1])
+
+
 @pytest.mark.parametrize("frame", [0, 1])
 @pytest.mark.parametrize("shape,roi", [((7, 8, 9), [1] * 6), ((20, 23, 31), [2, 3, 1, 4, 2, 1])])
 def test_tvtp_step_and_xbar(wctx, oracle, frame, shape, roi):

@@ -593,11 +593,263 @@ __global__ __launch_bounds__(1024) void k_flow_zm(
   }
 }
 
+// ---------------------------------------------------------------------------------
+// Two primal-dual iterations in one pass (both in the normal mode: no residual, no K^T y
+// kept). The intermediate state (v1, y1) never leaves the chip, so HBM sees the bytes
+// of one iteration for two. Each workgroup owns an output tile of (rw-3) x (rh-3)
+// columns and marches z over a chunk of zc planes; its rw x rh threads cover the tile
+// plus the halo the second iteration needs (one column/row below, two above):
+//   first primal  (v1)  on every thread         first dual (y1)  on lx <= rw-2, ly <= rh-2
+//   second primal (v2)  on 1 <= lx,ly <= rw-2    second dual (y2) on the tile (stored)
+// The march runs two planes behind the loads: step s computes the first primal at s+2,
+// the first dual and second primal at s+1 and the second dual at s. Neighbours in x and
+// y come through LDS (v0, v1 at s+1 and v2 at s before the first dual; the x and y
+// parts of y1 at s+1 before the second primal); the +x/+y neighbours' v1 at s were read
+// at the previous step and kept. Every value is formed by the same float operations as
+// in k_flow_zm, so the result equals two single iterations bit for bit.
+
+// primal step at one point from its duals y (all 10 at the point), the x parts at -x,
+// the y parts at -y and the z parts at -z, with the boundary rules of pd_point
+WAP_DEV void zm2_primal(const FlowGrid& G, int i, int j, int k, bool roi, const float* y,
+                        const float* ymx, const float* ymy, const float* ymz, const float* uu,
+                        const float* tau, const float* vin, float* vout) {
+  const bool xin = i > 0 && i < G.W - 1, yin = j > 0 && j < G.H - 1, zin = k > 0 && k < G.D - 1;
+  const float y4 = y[9];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const float y1 = y[3 * c], y2 = y[3 * c + 1], y3 = y[3 * c + 2];
+    const float dx = xin ? y1 - ymx[c] : (i == 0 ? y1 : -ymx[c]);
+    const float dy = yin ? y2 - ymy[c] : (j == 0 ? y2 : -ymy[c]);
+    const float dz = zin ? y3 - ymz[c] : (k == 0 ? y3 : -ymz[c]);
+    const float kt = -1.0f * dx - 1.0f * dy - 1.0f * dz + uu[c] * y4;
+    vout[c] = roi ? vin[c] - tau[c] * kt : vin[c];
+  }
+}
+
+// dual step (normal mode) at one point: nbn/nbo hold the new/old v of the +x, +y, +z
+// neighbours ([direction][component])
+template <class S>
+WAP_DEV void zm2_dual(const float* y, const float* uu, float utt, float s2, const float* vn,
+                      const float* vo, const float (*nbn)[3], const float (*nbo)[3], bool hx,
+                      bool hy, bool hz, float sigma1, float lambda, float shrink_den,
+                      float* yout) {
+  const bool has[3] = {hx, hy, hz};
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      const float kx = has[a] ? 1.0f * (nbn[a][c] - vn[c]) : 0.0f;
+      const float ko = has[a] ? 1.0f * (nbo[a][c] - vo[c]) : 0.0f;
+      const float yt = y[3 * c + a] + sigma1 * (kx + kx - ko);
+      yout[3 * c + a] = to_f(from_f<S>(yt * lambda / shrink_den));
+    }
+  }
+  const float ux = uu[0], uy = uu[1], uz = uu[2];
+  const float k4 = ux * vn[0] + uy * vn[1] + uz * vn[2];
+  const float k4o = ux * vo[0] + uy * vo[1] + uz * vo[2];
+  const float y4t = y[9] + s2 * (k4 + k4 - k4o + utt);
+  const float m1 = (y4t < 1.0f) ? y4t : 1.0f;
+  yout[9] = to_f(from_f<S>((-1.0f < m1) ? m1 : -1.0f));
+}
+
+// the loads of one plane of a column for the first primal step
+struct Zm2Raw {
+  float v0[3], y[10], ymx[3], ymy[3], u[3], ut;
+};
+// a plane after the first primal step; tau and s2 (functions of u alone) serve both
+// iterations
+struct Zm2Plane {
+  float v0[3], v1[3], y[10], u[3], ut, tau[3], s2;
+};
+
+template <class S, bool PRE>
+__global__ __launch_bounds__(1024) void k_flow_zm2(
+    FlowGrid G, const float* __restrict__ vin, float* __restrict__ vout,
+    const S* __restrict__ yin, S* __restrict__ yout, const S* __restrict__ u,
+    const S* __restrict__ ut, int rw, int rh, int zc, float sigma1, float lambda,
+    int* __restrict__ ctl, int iter) {
+  extern __shared__ float sh[];  // (9 + 6) floats per thread
+  if (ctl[0] == 0) return;
+  const int W = G.W, H = G.H, D = G.D;
+  const size_t n = (size_t)W * H * D;
+  const int nt = rw * rh, tid = (int)threadIdx.x;
+  const int lx = tid % rw, ly = tid / rw;
+  const int tw = rw - 3, th = rh - 3;
+  const unsigned tilesX = (unsigned)((W + tw - 1) / tw), tilesY = (unsigned)((H + th - 1) / th);
+  const unsigned bid = xcd_remap(blockIdx.x, gridDim.x);
+  const int x0 = (int)(bid % tilesX) * tw;
+  const int y0 = (int)((bid / tilesX) % tilesY) * th;
+  const int z0 = (int)(bid / (tilesX * tilesY)) * zc;
+  const int z1 = min(z0 + zc, D);
+  if (blockIdx.x == 0 && tid == 0) ctl[1] = iter + 1;
+  const int gi = x0 - 1 + lx, gj = y0 - 1 + ly;
+  // threads past the grid edge run on clamped coordinates; nothing valid reads them
+  const int i = iclamp(gi, 0, W - 1), j = iclamp(gj, 0, H - 1);
+  const bool keep = lx >= 1 && lx <= rw - 3 && ly >= 1 && ly <= rh - 3 && gi < W && gj < H;
+  const bool hx = i < W - 1, hy = j < H - 1;
+  const bool rxy = i >= G.r0 && i < W - G.r1 && j >= G.r2 && j < H - G.r3;
+  auto roi_at = [&](int k) { return rxy && k >= G.r4 && k < D - G.r5; };
+  const float shrink_den = lambda + 2.0f * sigma1;
+  const unsigned sy = (unsigned)W;
+  const unsigned colb = 4u * ((unsigned)i + sy * (unsigned)j);
+  const unsigned szb = 4u * sy * (unsigned)H;
+  const unsigned xmb = i > 0 ? 4u : 0u, ymb = j > 0 ? 4u * sy : 0u;
+  float* x1 = sh;           // v0, v1 at s+1; v2 at s
+  float* x2 = sh + 9 * nt;  // x and y parts of y1 at s+1
+  const int tpx = lx < rw - 1 ? tid + 1 : tid, tpy = ly < rh - 1 ? tid + rw : tid;
+  const int tmx = lx > 0 ? tid - 1 : tid, tmy = ly > 0 ? tid - rw : tid;
+
+  auto load_plane = [&](int p, Zm2Raw& R) {
+    const unsigned tb = colb + szb * (unsigned)iclamp(p, 0, D - 1);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      R.v0[c] = ldb(vin + c * n, tb);
+      R.y[3 * c] = lds(yin + (3 * c) * n, tb);
+      R.y[3 * c + 1] = lds(yin + (3 * c + 1) * n, tb);
+      R.y[3 * c + 2] = lds(yin + (3 * c + 2) * n, tb);
+      R.ymx[c] = lds(yin + (3 * c) * n, tb - xmb);
+      R.ymy[c] = lds(yin + (3 * c + 1) * n, tb - ymb);
+      R.u[c] = lds(u + c * n, tb);
+    }
+    R.y[9] = lds(yin + 9 * n, tb);
+    R.ut = lds(ut, tb);
+  };
+  // first primal step of plane p; ymz = the z parts of y0 at p-1
+  auto first_primal = [&](int p, const Zm2Raw& R, const float* ymz, Zm2Plane& P) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      P.v0[c] = R.v0[c];
+      P.u[c] = R.u[c];
+    }
+#pragma unroll
+    for (int q = 0; q < 10; ++q) P.y[q] = R.y[q];
+    P.ut = R.ut;
+    float tau[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) tau[c] = 1.0f / (4.0f / 1.0f + my_abs(R.u[c]));
+    if constexpr (PRE) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) P.tau[c] = tau[c];
+      P.s2 = 1.0f / (fabsf(R.u[0]) + fabsf(R.u[1]) + fabsf(R.u[2]));
+    }
+    zm2_primal(G, i, j, p, roi_at(p), R.y, R.ymx, R.ymy, ymz, R.u, tau, R.v0, P.v1);
+  };
+  // tau and s2 of a plane: kept (PRE) or formed again
+  auto tau_of = [&](const Zm2Plane& P, float* tau) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) tau[c] = PRE ? P.tau[c] : 1.0f / (4.0f / 1.0f + my_abs(P.u[c]));
+  };
+  auto s2_of = [&](const float* uu, float kept) {
+    return PRE ? kept : 1.0f / (fabsf(uu[0]) + fabsf(uu[1]) + fabsf(uu[2]));
+  };
+
+  Zm2Plane P1;  // plane s+1
+  {
+    float c3[3];
+    const unsigned tb = colb + szb * (unsigned)iclamp(z0 - 2, 0, D - 1);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) c3[c] = lds(yin + (3 * c + 2) * n, tb);
+    Zm2Raw R;
+    load_plane(z0 - 1, R);
+    first_primal(z0 - 1, R, c3, P1);
+  }
+  // plane s after the first dual and the second primal
+  float b_v1[3] = {0.0f, 0.0f, 0.0f}, b_v2[3] = {0.0f, 0.0f, 0.0f}, b_u[3] = {0.0f, 0.0f, 0.0f};
+  float b_y[10] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f}, b_ut = 0.0f;
+  float b_s2 = 0.0f;
+  float nv1[2][3] = {{0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}};  // +x, +y neighbours' v1 at s
+  for (int s = z0 - 2; s < z1; ++s) {
+    Zm2Plane P2;
+    {
+      Zm2Raw R;
+      load_plane(s + 2, R);
+      const float ymz[3] = {P1.y[2], P1.y[5], P1.y[8]};
+      first_primal(s + 2, R, ymz, P2);
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      x1[c * nt + tid] = P1.v0[c];
+      x1[(3 + c) * nt + tid] = P1.v1[c];
+      x1[(6 + c) * nt + tid] = b_v2[c];
+    }
+    __syncthreads();
+    float n0[3][3], n1[3][3], m1[3][3], m2[3][3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      n0[0][c] = x1[c * nt + tpx];
+      n0[1][c] = x1[c * nt + tpy];
+      n0[2][c] = P2.v0[c];
+      n1[0][c] = x1[(3 + c) * nt + tpx];
+      n1[1][c] = x1[(3 + c) * nt + tpy];
+      n1[2][c] = P2.v1[c];
+      m2[0][c] = x1[(6 + c) * nt + tpx];
+      m2[1][c] = x1[(6 + c) * nt + tpy];
+      m1[0][c] = nv1[0][c];
+      m1[1][c] = nv1[1][c];
+      m1[2][c] = P1.v1[c];
+    }
+    // first dual at s+1
+    float y1n[10];
+    zm2_dual<S>(P1.y, P1.u, P1.ut, s2_of(P1.u, P1.s2), P1.v1, P1.v0, n1, n0, hx, hy,
+                s + 1 < D - 1, sigma1, lambda, shrink_den, y1n);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      x2[c * nt + tid] = y1n[3 * c];
+      x2[(3 + c) * nt + tid] = y1n[3 * c + 1];
+    }
+    __syncthreads();
+    // second primal at s+1
+    float v2n[3];
+    {
+      float ymx[3], ymy[3];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        ymx[c] = x2[c * nt + tmx];
+        ymy[c] = x2[(3 + c) * nt + tmy];
+      }
+      const float ymz[3] = {b_y[2], b_y[5], b_y[8]};
+      float tau[3];
+      tau_of(P1, tau);
+      zm2_primal(G, i, j, s + 1, roi_at(s + 1), y1n, ymx, ymy, ymz, P1.u, tau, P1.v1, v2n);
+    }
+    // second dual at s, the output plane
+    if (s >= z0) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) m2[2][c] = v2n[c];
+      float y2[10];
+      zm2_dual<S>(b_y, b_u, b_ut, s2_of(b_u, b_s2), b_v2, b_v1, m2, m1, hx, hy, s < D - 1,
+                  sigma1, lambda, shrink_den, y2);
+      if (keep) {
+        const unsigned tb = colb + szb * (unsigned)s;
+#pragma unroll
+        for (int q = 0; q < 10; ++q) sts(yout + q * n, tb, y2[q]);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) stb(vout + c * n, tb, b_v2[c]);
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      nv1[0][c] = n1[0][c];
+      nv1[1][c] = n1[1][c];
+      b_v1[c] = P1.v1[c];
+      b_v2[c] = v2n[c];
+      b_u[c] = P1.u[c];
+    }
+#pragma unroll
+    for (int q = 0; q < 10; ++q) b_y[q] = y1n[q];
+    b_ut = P1.ut;
+    if (PRE) b_s2 = P1.s2;
+    P1 = P2;
+  }
+}
+
 // err = (d*d + p*p)/nPx in float from the float-rounded sums; stop when !(err > tol)
+// (flips: buffer swaps the host has queued up to and including this iteration; kept at a
+// stop so the settle step knows which buffers hold the state)
 __global__ __launch_bounds__(256) void k_flow_resid(const double* __restrict__ ppart, int np,
                                                     const double* __restrict__ dpart, int nd,
                                                     int npx, float tol, int* __restrict__ ctl,
-                                                    float* __restrict__ errout) {
+                                                    float* __restrict__ errout, int flips) {
   if (ctl[0] == 0) return;
   double ps = 0.0, ds = 0.0;
   for (int q = threadIdx.x; q < np; q += blockDim.x) ps += ppart[q];
@@ -609,20 +861,22 @@ __global__ __launch_bounds__(256) void k_flow_resid(const double* __restrict__ p
     const float p = (float)P, d = (float)Dd;
     const float err = (d * d + p * p) / (float)npx;
     if (errout) *errout = err;
-    if (!(err > tol)) ctl[0] = 0;
+    if (!(err > tol)) {
+      ctl[0] = 0;
+      ctl[2] = flips;
+    }
   }
 }
 
-// After a warp: the host alternated the v and y buffers for every queued iteration;
-// if the device stopped early with the other parity, move the live state into the
-// buffers the host treats as current.
+// After a warp: the host swapped the v and y buffers once per queued launch (host_flips
+// swaps in all); if the device stopped early after a number of swaps of the other
+// parity, move the live state into the buffers the host treats as current.
 template <class S>
-__global__ void k_flow_settle(size_t n, int host_last, float* __restrict__ v_host,
+__global__ void k_flow_settle(size_t n, int host_flips, float* __restrict__ v_host,
                               const float* __restrict__ v_other, S* __restrict__ y_host,
                               const S* __restrict__ y_other, const int* __restrict__ ctl) {
-  const int dev_last = ctl[1];
   const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (!((dev_last ^ host_last) & 1) || t >= n) return;
+  if (ctl[0] != 0 || !((ctl[2] ^ host_flips) & 1) || t >= n) return;
 #pragma unroll
   for (int c = 0; c < 3; ++c) v_host[c * n + t] = v_other[c * n + t];
 #pragma unroll
@@ -658,6 +912,7 @@ __global__ void k_zero_tail(double* __restrict__ p, size_t n) {
 __global__ void k_flow_arm(int* ctl) {
   ctl[0] = 1;
   ctl[1] = 0;
+  ctl[2] = 0;
 }
 
 }  // namespace
@@ -890,18 +1145,59 @@ hipError_t wapk_flow_pd(int variant, int storage, const int* dims, const int* ro
 }
 
 hipError_t wapk_flow_resid(const double* ppart, int np, const double* dpart, int nd, int npx,
-                           float tol, int* ctl, float* errout, hipStream_t s) {
+                           float tol, int* ctl, float* errout, int flips, hipStream_t s) {
   hipLaunchKernelGGL(k_flow_resid, dim3(1), dim3(256), 0, s, ppart, np, dpart, nd, npx, tol, ctl,
-                     errout);
+                     errout, flips);
   return hipGetLastError();
 }
 
-hipError_t wapk_flow_settle(int storage, size_t n, int host_last, float* v_host,
+hipError_t wapk_flow_settle(int storage, size_t n, int host_flips, float* v_host,
                             const float* v_other, void* y_host, const void* y_other,
                             const int* ctl, hipStream_t s) {
   WAP_BY_STORAGE(storage, hipLaunchKernelGGL(k_flow_settle<S>, dim3((unsigned)((n + 255) / 256)),
-                                             dim3(256), 0, s, n, host_last, v_host, v_other,
+                                             dim3(256), 0, s, n, host_flips, v_host, v_other,
                                              (S*)y_host, (const S*)y_other, ctl));
+  return hipGetLastError();
+}
+
+// Shapes of the two-iteration pass: rw x rh threads (output tile (rw-3) x (rh-3)) and
+// zc planes per workgroup.
+struct Zm2Shape {
+  int rw, rh, zc;
+};
+static const Zm2Shape kZm2[] = {{35, 29, 32}, {35, 29, 16}, {67, 15, 32}, {67, 15, 16},
+                                {35, 18, 32}, {64, 16, 32}, {131, 7, 32},  {19, 53, 32},
+                                {35, 29, 64}, {19, 27, 16}, {32, 16, 32}, {32, 16, 16},
+                                {64, 8, 32},  {64, 8, 16},  {35, 14, 32}, {35, 14, 16},
+                                {16, 32, 32}, {67, 7, 32}};
+static const int kZm2Count = (int)(sizeof(kZm2) / sizeof(kZm2[0]));
+
+// forms 0..kZm2Count-1 form tau and s2 again per iteration; the next kZm2Count keep them
+int wapk_flow_pair_shapes() { return 2 * kZm2Count; }
+
+// Two iterations (it, it+1) in the normal mode from (vold, yold) into (vnew, ynew).
+hipError_t wapk_flow_pd2(int shape, int storage, const int* dims, const int* roi,
+                         const float* vold, float* vnew, const void* yold, void* ynew,
+                         const void* u, const void* ut, float sigma1, float lambda, int* ctl,
+                         int iter, hipStream_t s) {
+  if (shape < 0 || shape >= 2 * kZm2Count) return hipErrorInvalidValue;
+  const size_t n = (size_t)dims[0] * dims[1] * dims[2];
+  if (n >= (1ull << 30)) return hipErrorInvalidValue;  // 32-bit in-plane byte offsets
+  const bool pre = shape >= kZm2Count;
+  const Zm2Shape z = kZm2[shape % kZm2Count];
+  const int tw = z.rw - 3, th = z.rh - 3;
+  const size_t nb = (size_t)((dims[0] + tw - 1) / tw) * ((dims[1] + th - 1) / th) *
+                    ((dims[2] + z.zc - 1) / z.zc);
+  const size_t ldsz = sizeof(float) * 15 * z.rw * z.rh;
+#define WAP_ZM2(PRE)                                                                         \
+  WAP_BY_STORAGE(storage, hipLaunchKernelGGL((k_flow_zm2<S, PRE>), dim3((unsigned)nb),         \
+                                             dim3(z.rw * z.rh), ldsz, s, mkgrid(dims, roi),     \
+                                             vold, vnew, (const S*)yold, (S*)ynew, (const S*)u, \
+                                             (const S*)ut, z.rw, z.rh, z.zc, sigma1, lambda,    \
+                                             ctl, iter))
+  if (pre) WAP_ZM2(true);
+  else WAP_ZM2(false);
+#undef WAP_ZM2
   return hipGetLastError();
 }
 

@@ -46,7 +46,10 @@ hipError_t wapk_flow_pd(int, int, const int*, const int*, const float*, float*, 
                         void*, const void*, const void*, const float*, float*, int, int, int,
                         float, float, double*, double*, int*, int, hipStream_t);
 hipError_t wapk_flow_resid(const double*, int, const double*, int, int, float, int*, float*,
-                           hipStream_t);
+                           int, hipStream_t);
+int wapk_flow_pair_shapes();
+hipError_t wapk_flow_pd2(int, int, const int*, const int*, const float*, float*, const void*,
+                         void*, const void*, const void*, float, float, int*, int, hipStream_t);
 hipError_t wapk_flow_settle(int, size_t, int, float*, const float*, void*, const void*,
                             const int*, hipStream_t);
 hipError_t wapk_flow_arm(int*, hipStream_t);
@@ -97,6 +100,10 @@ struct WapContext {
   std::map<std::array<int, 4>, int> flow_tuned;  // autotuner choices by (W, H, D, storage)
   int flow_storage = 0;  // storage of the flow duals, gradient and ut (WapDType)
   int flow_last = 0;                              // form used by the last flow level
+  // two-iterations-per-pass form: -1 = autotuned per grid size, 0 = off, k = shape k-1
+  int flow_pair = -1;
+  std::map<std::array<int, 4>, int> pair_tuned;  // chosen shape (or -1 = off) by size
+  int flow_pair_last = -1;
   int project_unroll = 2;                         // ray samples per march trip
   int project_mapping = -1;  // lanes-to-rays mapping of the projector; -1 = per view
   std::vector<WapView> host_views;                // host copy of the view table
@@ -428,6 +435,61 @@ int flow_tune(WapContext* c, const int* dims, const int* roi, const float* vcur,
   return WAP_OK;
 }
 
+// Autotuner of the two-iterations-per-pass form: times each shape against two launches
+// of the chosen single-iteration form on the level's own buffers (as flow_tune) and keeps
+// the fastest shape, or none when the single form is faster at this size.
+int pair_tune(WapContext* c, const int* dims, const int* roi, const float* vcur, float* vnext,
+              const void* ycur, void* ynext, const void* u, const void* ut, const float* kx4,
+              float* kty, int* ctl, hipStream_t st, int variant, int* chosen) {
+  const std::array<int, 4> key = {dims[0], dims[1], dims[2], c->flow_storage};
+  auto it = c->pair_tuned.find(key);
+  if (it != c->pair_tuned.end()) {
+    *chosen = it->second;
+    return WAP_OK;
+  }
+  hipEvent_t e0, e1;
+  WAP_HIP(c, hipEventCreate(&e0));
+  WAP_HIP(c, hipEventCreate(&e1));
+  int best = -1;
+  float best_ms = 1e30f;
+  int rc = WAP_OK;
+  for (int sh = -1; sh < wapk_flow_pair_shapes(); ++sh) {
+    float ms = 0.0f;
+    hipError_t e = wapk_flow_arm(ctl, st);
+    for (int r = 0; r < 4 && e == hipSuccess; ++r) {
+      if (r == 1) e = hipEventRecord(e0, st);
+      if (e != hipSuccess) break;
+      if (sh < 0) {
+        // the baseline: two single iterations (both read the state and write the other
+        // buffers, so the state is untouched)
+        for (int k = 0; k < 2 && e == hipSuccess; ++k)
+          e = wapk_flow_pd(variant, c->flow_storage, dims, roi, vcur, vnext, ycur, ynext, u, ut,
+                           kx4, kty, 2, 0, 0, 1.0f / 3.0f, 0.05f, nullptr, nullptr, ctl, 2, st);
+      } else {
+        e = wapk_flow_pd2(sh, c->flow_storage, dims, roi, vcur, vnext, ycur, ynext, u, ut,
+                          1.0f / 3.0f, 0.05f, ctl, 2, st);
+      }
+    }
+    if (e == hipSuccess) e = hipEventRecord(e1, st);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    if (e != hipSuccess) {
+      rc = fail(c, WAP_E_HIP, std::string("flow pair autotuner: ") + hipGetErrorString(e));
+      break;
+    }
+    if (ms < best_ms) {
+      best_ms = ms;
+      best = sh;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc != WAP_OK) return rc;
+  c->pair_tuned[key] = best;
+  *chosen = best;
+  return WAP_OK;
+}
+
 int flow_level(WapContext* c, const int* dims, const float* I1, const float* I2, float* V,
                float* Y, float tol, float lambda, int max_iter, int nwarps, const float* aabb,
                const int* roi, int* iters_run, hipStream_t st) {
@@ -468,6 +530,10 @@ int flow_level(WapContext* c, const int* dims, const float* I1, const float* I2,
   if (variant < 0)
     WAP_TRY(flow_tune(c, dims, roi, V, vB, yA, yB, u, ut, kx4, kty, ctl, st, &variant));
   c->flow_last = variant;
+  int pair = c->flow_pair > 0 ? c->flow_pair - 1 : -1;
+  if (c->flow_pair < 0)
+    WAP_TRY(pair_tune(c, dims, roi, V, vB, yA, yB, u, ut, kx4, kty, ctl, st, variant, &pair));
+  c->flow_pair_last = pair;
   const size_t nblk = wapk_flow_blocks(variant, dims);
   WAP_TRY(scratch(c, "flow_pp", nblk, &pp));
   WAP_TRY(scratch(c, "flow_dp", nblk, &dp));
@@ -480,19 +546,35 @@ int flow_level(WapContext* c, const int* dims, const float* I1, const float* I2,
     if (w > 0) WAP_HIP(c, wapk_flow_kx4(sto, dims, u, vcur, kx4, st));
     WAP_TRY(advect_impl(c, dims, aabb, vcur, I2, 1, -1.0f, warped, st));
     if (has_roi) WAP_HIP(c, wapk_flow_prep(sto, dims, roi, warped, I1, vcur, u, ut, st));
-    const int last = max_iter + 1;
-    for (int it = 1; it <= last; ++it) {
+    // the reference's loop starts from err = 1 and runs while err > tol: a tolerance of 1
+    // or more (or NaN) runs no iteration
+    const int last = 1.0f > tol ? max_iter + 1 : 0;
+    // launches swap the buffers once each; two normal-mode iterations that neither keep
+    // K^T y nor sum the residual run as one pass when the pair form is on
+    int flips = 0;
+    for (int it = 1; it <= last;) {
       const int mode = (it == 1) ? (w == 0 ? 0 : 1) : 2;
       const int resid = (it % 50 == 0);
       const int store = (it % 50 == 49);
-      WAP_HIP(c, wapk_flow_pd(variant, sto, dims, roi, vcur, vnext, ycur, ynext, u, ut, kx4, kty,
-                              mode, resid, store, sigma1, lambda, pp, dp, ctl, it, st));
-      if (resid)
-        WAP_HIP(c, wapk_flow_resid(pp, (int)nblk, dp, (int)nblk, (int)n, tol, ctl, err, st));
+      const bool two = pair >= 0 && mode == 2 && it + 1 <= last && !resid && !store &&
+                       (it + 1) % 50 != 0 && (it + 1) % 50 != 49;
+      if (two) {
+        WAP_HIP(c, wapk_flow_pd2(pair, sto, dims, roi, vcur, vnext, ycur, ynext, u, ut, sigma1,
+                                 lambda, ctl, it, st));
+        it += 2;
+      } else {
+        WAP_HIP(c, wapk_flow_pd(variant, sto, dims, roi, vcur, vnext, ycur, ynext, u, ut, kx4,
+                                kty, mode, resid, store, sigma1, lambda, pp, dp, ctl, it, st));
+        if (resid)
+          WAP_HIP(c, wapk_flow_resid(pp, (int)nblk, dp, (int)nblk, (int)n, tol, ctl, err,
+                                     flips + 1, st));
+        it += 1;
+      }
+      ++flips;
       std::swap(vcur, vnext);
       std::swap(ycur, ynext);
     }
-    WAP_HIP(c, wapk_flow_settle(sto, n, last, vcur, vnext, ycur, ynext, ctl, st));
+    WAP_HIP(c, wapk_flow_settle(sto, n, flips, vcur, vnext, ycur, ynext, ctl, st));
     if (iters_run) {
       WAP_HIP(c, hipMemcpyAsync(ctl_host.data(), ctl, sizeof(int) * 2, hipMemcpyDeviceToHost, st));
       WAP_HIP(c, hipStreamSynchronize(st));
@@ -684,8 +766,15 @@ int wap_set_tuning(WapContext* c, const char* key, int value) {
     c->project_unroll = value;
     return WAP_OK;
   }
+  if (strcmp(key, "flow_pair") == 0) {
+    if (value < -1 || value > wapk_flow_pair_shapes())
+      return fail(c, WAP_E_ARG, "flow_pair out of range");
+    c->flow_pair = value;
+    return WAP_OK;
+  }
   if (strcmp(key, "flow_retune") == 0) {
     c->flow_tuned.clear();
+    c->pair_tuned.clear();
     return WAP_OK;
   }
   return fail(c, WAP_E_ARG, std::string("unknown tuning key ") + key);
@@ -719,6 +808,18 @@ int wap_get_tuning(const WapContext* c, const char* key, int* value) {
   }
   if (strcmp(key, "flow_variant_count") == 0) {
     *value = wapk_flow_variants();
+    return WAP_OK;
+  }
+  if (strcmp(key, "flow_pair") == 0) {
+    *value = c->flow_pair;
+    return WAP_OK;
+  }
+  if (strcmp(key, "flow_pair_used") == 0) {
+    *value = c->flow_pair_last + 1;
+    return WAP_OK;
+  }
+  if (strcmp(key, "flow_pair_count") == 0) {
+    *value = wapk_flow_pair_shapes();
     return WAP_OK;
   }
   return WAP_E_ARG;
