@@ -46,6 +46,12 @@ def parse():
                         "calibrates the HBM byte counters (use under rocprofv3)")
     p.add_argument("--pd-form", type=int, default=-1,
                    help="with --profile-pd: the PD iteration form (-1 = autotuned)")
+    p.add_argument("--pd-pair", type=int, default=-1,
+                   help="with --profile-pd: the pair-pass shape (-1 = autotuned, 0 = off)")
+    p.add_argument("--profile-proj", action="store_true",
+                   help="run only the profiling workload of the forward projector: one foam "
+                        "state at full size projected along 8 views, then the same calibration "
+                        "copy as --profile-pd (use under rocprofv3)")
     p.add_argument("--no-forms", action="store_true",
                    help="skip the precision-form step (half storage of the flow state)")
     return p.parse_args()
@@ -87,12 +93,60 @@ def make_workload(ctx, W, L, S, NDET, torch, dev):
 
 ZM_NAME = "k_flow_zm (single-pass primal-dual iteration of the L1 flow)"
 TWO_NAME = "k_flow_primal + k_flow_dual (primal-dual iteration of the L1 flow)"
+PAIR_NAME = "k_flow_zm2 (two primal-dual iterations of the L1 flow per pass)"
+# algorithmic bytes of one PD voxel-iteration: read v, v_old (the dual's K v_old term),
+# y (10), u (3), ut; write v (3), y (10) = 33 floats
+PD_BYTES = 132
+# compulsory HBM bytes of one pair pass per voxel: the state is read and written once
+# for the two iterations (read v 3, y 10, u 3, ut 1; write v 3, y 10 = 30 floats)
+PAIR_BYTES = 120
+
+
+def _solve_timer(ctx, torch, dev, L):
+    I1 = torch.rand((L, L, L), device=dev)
+    I2 = torch.rand((L, L, L), device=dev)
+    V = torch.zeros((3, L, L, L), device=dev)
+    Y = torch.zeros((10, L, L, L), device=dev)
+    aabb = [0, L, 0, L, 0, L]
+    roi = [8] * 6
+
+    def run(iters):
+        V.zero_()
+        Y.zero_()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        ctx.flow_solve(I1, I2, V, Y, 0.0, 0.05, iters, 1, aabb, roi)
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1)
+    return run
+
+
+def pd_pair_ms(ctx, torch, dev, L):
+    """Average duration of one pass of the two-iteration kernel at full resolution (the
+    shape the autotuner picks): a solve of 46 iterations queues one single iteration and
+    23 pair passes, one of 4 iterations one single and 2 pairs, so their difference is 21
+    pair passes and nothing else. Timed with events on the stream the kernels run on
+    (torch's current stream). Returns (ms, shape) or (None, 0) when the autotuner keeps
+    single iterations at this size."""
+    run = _solve_timer(ctx, torch, dev, L)
+    run(4)
+    shape = ctx.get_tuning("flow_pair_used")
+    if shape == 0:
+        return None, 0
+    t_lo = min(run(4) for _ in range(2))
+    t_hi = min(run(46) for _ in range(2))
+    return (t_hi - t_lo) / 21.0, shape
 
 
 def pd_iteration_ms(ctx, torch, dev, L, storage=0):
-    """Average duration of one flow primal-dual iteration at full resolution (the form the
-    autotuner picks), from two solves that differ only in the iteration count, timed with
-    events on the stream the kernels run on (torch's current stream)."""
+    """Average duration of one single flow primal-dual iteration at full resolution (the
+    form the autotuner picks, pair passes off), from two solves that differ only in the
+    iteration count, timed with events on the stream the kernels run on (torch's current
+    stream)."""
+    pair = ctx.get_tuning("flow_pair")
+    ctx.set_tuning("flow_pair", 0)
     ctx.set_tuning("flow_storage", storage)
     I1 = torch.rand((L, L, L), device=dev)
     I2 = torch.rand((L, L, L), device=dev)
@@ -112,39 +166,59 @@ def pd_iteration_ms(ctx, torch, dev, L, storage=0):
         torch.cuda.synchronize()
         return e0.elapsed_time(e1)
 
+    run = _solve_timer(ctx, torch, dev, L)
     run(5)
     t_lo = min(run(10) for _ in range(2))
     t_hi = min(run(60) for _ in range(2))
     variant = ctx.get_tuning("flow_variant_used")
     ctx.set_tuning("flow_storage", 0)
+    ctx.set_tuning("flow_pair", pair)
     return (t_hi - t_lo) / 50.0, variant
 
 
-def measured_traffic(variant, L):
-    """HBM bytes per launch of the PD iteration from the committed rocprofv3 counter
-    summary (FETCH_SIZE and WRITE_SIZE passes scaled by a copy of known size), when it
-    was taken for this same form and size; otherwise None."""
+def measured_traffic(key, L):
+    """HBM bytes per launch of a PD kernel from the committed rocprofv3 counter summary
+    (FETCH_SIZE and WRITE_SIZE passes scaled by a copy of known size), when it was taken
+    for this same form and size; otherwise None. key: "form_<v>" (single iteration) or
+    "pair_<shape>" (two-iteration pass)."""
     path = os.path.join(ROOT, "profiles", f"flow_pd_{L}_traffic.txt")
     if not os.path.exists(path):
         return None
     for line in open(path):
-        if line.startswith(f"form_{variant}_traffic_bytes_per_launch:"):
+        if line.startswith(f"{key}_traffic_bytes_per_launch:"):
             return int(float(line.split(":", 1)[1]))
     return None
 
 
 def roofline_pd(ctx, torch, dev, L):
-    per_ms, variant = pd_iteration_ms(ctx, torch, dev, L)
+    """Roofline of the dominant kernel, the flow's primal-dual iteration. Algorithmic
+    bytes per launch = 132 B per voxel-iteration x the voxel-iterations of one launch (n
+    for a single iteration, 2n for a pair pass)."""
     n = L * L * L
-    # algorithmic bytes per voxel-iteration: read v (3), v_old (3), y (10), u (4);
-    # write v (3), y (10) = 33 floats = 132 B
-    algo = 132 * n
-    ach = algo / (per_ms * 1e-3) / 1e9
+    per_ms, variant = pd_iteration_ms(ctx, torch, dev, L)
+    single = {"kernel": (TWO_NAME if variant == 0 else ZM_NAME) + f", form {variant}",
+              "ms_per_launch": round(per_ms, 4), "bytes_per_launch": PD_BYTES * n,
+              "achieved": round(PD_BYTES * n / (per_ms * 1e-3) / 1e9, 1),
+              "traffic": measured_traffic(f"form_{variant}", L)}
+    single["frac"] = round(single["achieved"] / HBM_PEAK_GBS, 4)
+    pair_ms, shape = pd_pair_ms(ctx, torch, dev, L)
+    if pair_ms is None:
+        out = {"bound": "hbm", "achieved": single["achieved"], "peak": HBM_PEAK_GBS,
+               "unit": "GB/s", "frac": single["frac"], "traffic": single["traffic"]}
+        out.update({k: single[k] for k in ("kernel", "ms_per_launch", "bytes_per_launch")})
+        return out
+    algo = 2 * PD_BYTES * n
+    ach = algo / (pair_ms * 1e-3) / 1e9
+    comp = PAIR_BYTES * n / (pair_ms * 1e-3) / 1e9
     return {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
-            "traffic": measured_traffic(variant, L),
-            "kernel": (TWO_NAME if variant == 0 else ZM_NAME) + f", form {variant}",
-            "ms_per_launch": round(per_ms, 4), "bytes_per_launch": algo}
+            "traffic": measured_traffic(f"pair_{shape}", L),
+            "kernel": PAIR_NAME + f", shape {shape}",
+            "ms_per_launch": round(pair_ms, 4), "bytes_per_launch": algo,
+            "ms_per_iteration": round(pair_ms / 2.0, 4),
+            "compulsory_bytes_per_launch": PAIR_BYTES * n,
+            "compulsory_gbs": round(comp, 1),
+            "single_iteration": single}
 
 
 def projector_rate(ctx, torch, vols, sd, S, NDET, L):
@@ -230,10 +304,12 @@ def main():
     ctx = W.Context(0)
     if a.profile_pd:
         ctx.set_tuning("flow_variant", a.pd_form)
+        ctx.set_tuning("flow_pair", a.pd_pair)
         I1 = torch.rand((L, L, L), device=dev)
         I2 = torch.rand((L, L, L), device=dev)
         V = torch.zeros((3, L, L, L), device=dev)
         Y = torch.zeros((10, L, L, L), device=dev)
+        # 8 iterations: one single, then four pair passes (when pairs are on)
         ctx.flow_solve(I1, I2, V, Y, 0.0, 0.05, 8, 1, [0, L, 0, L, 0, L], [8] * 6)
         torch.cuda.synchronize()
         src = torch.rand(13 * L ** 3, device=dev)
@@ -242,6 +318,25 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"profile_pd": True, "size": L,
                           "form": ctx.get_tuning("flow_variant_used"),
+                          "pair_shape": ctx.get_tuning("flow_pair_used"),
+                          "copy_bytes_each_way": src.numel() * 4}))
+        ctx.close()
+        return
+    if a.profile_proj:
+        import phantoms
+        rng = phantoms.SplitMix64(phantoms.SEED_BASE + 2)
+        vol = torch.from_numpy(phantoms.foam_phantom(L, 400, rng)).to(dev)
+        geo = W.make_geometry((L, L, L), NDET, NDET, ds=1.0, sid=4.0 * L, sdd=6.0 * L)
+        ctx.upload_views(W.views(geo, phantoms.golden_angles(8, 360.0)))
+        proj = torch.empty((8, NDET, NDET), device=dev)
+        sd = float(np.float32(math.sqrt(3.0) / 2.0))
+        ctx.forward_project(vol, sd, 0, 8, NDET, NDET, proj)
+        torch.cuda.synchronize()
+        src = torch.rand(13 * L ** 3, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"profile_proj": True, "size": L, "views": 8, "detector": NDET,
                           "copy_bytes_each_way": src.numel() * 4}))
         ctx.close()
         return

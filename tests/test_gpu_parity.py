@@ -279,30 +279,46 @@ PAIR_CASES = [((20, 18, 70), [3, 2, 1, 2, 2, 3]), ((14, 16, 18), [2] * 6),
               ((37, 11, 67), [1, 0, 2, 1, 0, 3]), ((70, 40, 37), [4, 1, 3, 0, 2, 5])]
 
 
-@pytest.mark.parametrize("tol", [0.008, 0.3, 0.9, 1e9])
-@pytest.mark.parametrize("shape_id", list(range(1, 11)))
-def test_flow_pair_form_identical(wctx, oracle, shape_id, tol):
-    """The two-iterations-per-pass form, each tile shape forced, gives the oracle's floats
-    (v, all ten duals and the iteration count), also when the residual stops a warp early
-    (tol 0.3 and 0.9 stop warps at iteration 50 or 100, between queued pairs) and when the
+_PAIR_REF = {}
+
+
+def _pair_reference(oracle, case, tol):
+    """Oracle solve of one PAIR_CASES grid at one tolerance (cached across shapes)."""
+    key = (case, tol)
+    if key not in _PAIR_REF:
+        shape, roi = PAIR_CASES[case]
+        I1, I2, aabb = _flow_case(oracle, shape, roi, 13)
+        z3 = np.zeros((3,) + shape, np.float32)
+        z1 = np.zeros(shape, np.float32)
+        v, y1, y2, y3, y4, it = oracle.flow_single(I1, I2, z3, z3, z3, z3, z1, tol, 0.05, 120,
+                                                   2, aabb, roi)
+        _PAIR_REF[key] = (I1, I2, aabb, v, np.concatenate([y1, y2, y3, y4[None]]), it)
+    return _PAIR_REF[key]
+
+
+@pytest.mark.parametrize("shape_id", list(range(1, 49)))
+def test_flow_pair_form_identical(wctx, oracle, shape_id):
+    """The two-iterations-per-pass form, each tile shape forced (plain, with tau and s2
+    kept per plane, and prefetching), gives the oracle's floats (v, all ten duals and the
+    iteration count) on four grids, also when the residual stops a warp early (tol 0.3
+    and 0.9 stop warps at iteration 50 or 100, between queued pairs) and when the
     reference's initial err = 1 runs no iteration at all (tol 1e9)."""
+    assert wctx.get_tuning("flow_pair_count") == 48
     prev = wctx.get_tuning("flow_pair")
     wctx.set_tuning("flow_pair", shape_id)
     try:
-        for shape, roi in PAIR_CASES:
-            I1, I2, aabb = _flow_case(oracle, shape, roi, 13)
-            z3 = np.zeros((3,) + shape, np.float32)
-            z1 = np.zeros(shape, np.float32)
-            V = torch.zeros((3,) + shape, device=DEV)
-            Y = torch.zeros((10,) + shape, device=DEV)
-            print(f"pair shape {shape_id} grid {shape} roi {roi} tol {tol}", flush=True)
-            it = wctx.flow_solve(T(I1), T(I2), V, Y, tol, 0.05, 120, 2, aabb, roi, count=True)
-            assert wctx.get_tuning("flow_pair_used") == shape_id
-            v, y1, y2, y3, y4, it_ref = oracle.flow_single(I1, I2, z3, z3, z3, z3, z1, tol, 0.05,
-                                                           120, 2, aabb, roi)
-            assert it == it_ref
-            np.testing.assert_array_equal(H(V), v)
-            np.testing.assert_array_equal(H(Y), np.concatenate([y1, y2, y3, y4[None]]))
+        for case, (shape, roi) in enumerate(PAIR_CASES):
+            for tol in (0.008, 0.3, 0.9, 1e9):
+                I1, I2, aabb, v, y, it_ref = _pair_reference(oracle, case, tol)
+                V = torch.zeros((3,) + shape, device=DEV)
+                Y = torch.zeros((10,) + shape, device=DEV)
+                print(f"pair shape {shape_id} grid {shape} roi {roi} tol {tol}", flush=True)
+                it = wctx.flow_solve(T(I1), T(I2), V, Y, tol, 0.05, 120, 2, aabb, roi,
+                                     count=True)
+                assert wctx.get_tuning("flow_pair_used") == shape_id
+                assert it == it_ref
+                np.testing.assert_array_equal(H(V), v)
+                np.testing.assert_array_equal(H(Y), y)
     finally:
         wctx.set_tuning("flow_pair", prev)
 

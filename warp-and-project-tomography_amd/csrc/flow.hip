@@ -662,8 +662,11 @@ struct Zm2Plane {
   float v0[3], v1[3], y[10], u[3], ut, tau[3], s2;
 };
 
-template <class S, bool PRE>
-__global__ __launch_bounds__(1024) void k_flow_zm2(
+// PF: the loads of plane s+3 are issued before step s computes and used at step s+1,
+// so they are in flight over a whole step (the kernel then takes at most 768 threads,
+// for the registers the loaded plane needs).
+template <class S, bool PRE, bool PF>
+__global__ __launch_bounds__(PF ? 768 : 1024) void k_flow_zm2(
     FlowGrid G, const float* __restrict__ vin, float* __restrict__ vout,
     const S* __restrict__ yin, S* __restrict__ yout, const S* __restrict__ u,
     const S* __restrict__ ut, int rw, int rh, int zc, float sigma1, float lambda,
@@ -758,13 +761,22 @@ __global__ __launch_bounds__(1024) void k_flow_zm2(
   float b_y[10] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f}, b_ut = 0.0f;
   float b_s2 = 0.0f;
   float nv1[2][3] = {{0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}};  // +x, +y neighbours' v1 at s
+  Zm2Raw Rc;  // PF: the loads of plane s+2
+  if (PF) load_plane(z0, Rc);
   for (int s = z0 - 2; s < z1; ++s) {
     Zm2Plane P2;
     {
-      Zm2Raw R;
-      load_plane(s + 2, R);
       const float ymz[3] = {P1.y[2], P1.y[5], P1.y[8]};
-      first_primal(s + 2, R, ymz, P2);
+      if constexpr (PF) {
+        Zm2Raw Rn;
+        load_plane(s + 3, Rn);
+        first_primal(s + 2, Rc, ymz, P2);
+        Rc = Rn;
+      } else {
+        Zm2Raw R;
+        load_plane(s + 2, R);
+        first_primal(s + 2, R, ymz, P2);
+      }
     }
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
@@ -1171,32 +1183,40 @@ static const Zm2Shape kZm2[] = {{35, 29, 32}, {35, 29, 16}, {67, 15, 32}, {67, 1
                                 {64, 8, 32},  {64, 8, 16},  {35, 14, 32}, {35, 14, 16},
                                 {16, 32, 32}, {67, 7, 32}};
 static const int kZm2Count = (int)(sizeof(kZm2) / sizeof(kZm2[0]));
+// shapes of the prefetching form (at most 768 threads)
+static const Zm2Shape kZm2P[] = {{35, 21, 32}, {35, 21, 16}, {67, 11, 32}, {67, 11, 16},
+                                 {32, 24, 32}, {64, 12, 32}, {35, 14, 32}, {35, 14, 16},
+                                 {32, 16, 32}, {64, 8, 32},  {19, 40, 32}, {35, 21, 64}};
+static const int kZm2PCount = (int)(sizeof(kZm2P) / sizeof(kZm2P[0]));
 
-// forms 0..kZm2Count-1 form tau and s2 again per iteration; the next kZm2Count keep them
-int wapk_flow_pair_shapes() { return 2 * kZm2Count; }
+// forms 0..kZm2Count-1 form tau and s2 again per iteration, the next kZm2Count keep them,
+// the last kZm2PCount prefetch the next plane
+int wapk_flow_pair_shapes() { return 2 * kZm2Count + kZm2PCount; }
 
 // Two iterations (it, it+1) in the normal mode from (vold, yold) into (vnew, ynew).
 hipError_t wapk_flow_pd2(int shape, int storage, const int* dims, const int* roi,
                          const float* vold, float* vnew, const void* yold, void* ynew,
                          const void* u, const void* ut, float sigma1, float lambda, int* ctl,
                          int iter, hipStream_t s) {
-  if (shape < 0 || shape >= 2 * kZm2Count) return hipErrorInvalidValue;
+  if (shape < 0 || shape >= wapk_flow_pair_shapes()) return hipErrorInvalidValue;
   const size_t n = (size_t)dims[0] * dims[1] * dims[2];
   if (n >= (1ull << 30)) return hipErrorInvalidValue;  // 32-bit in-plane byte offsets
-  const bool pre = shape >= kZm2Count;
-  const Zm2Shape z = kZm2[shape % kZm2Count];
+  const bool pf = shape >= 2 * kZm2Count;
+  const bool pre = !pf && shape >= kZm2Count;
+  const Zm2Shape z = pf ? kZm2P[shape - 2 * kZm2Count] : kZm2[shape % kZm2Count];
   const int tw = z.rw - 3, th = z.rh - 3;
   const size_t nb = (size_t)((dims[0] + tw - 1) / tw) * ((dims[1] + th - 1) / th) *
                     ((dims[2] + z.zc - 1) / z.zc);
   const size_t ldsz = sizeof(float) * 15 * z.rw * z.rh;
-#define WAP_ZM2(PRE)                                                                         \
-  WAP_BY_STORAGE(storage, hipLaunchKernelGGL((k_flow_zm2<S, PRE>), dim3((unsigned)nb),         \
+#define WAP_ZM2(PRE, PF)                                                                     \
+  WAP_BY_STORAGE(storage, hipLaunchKernelGGL((k_flow_zm2<S, PRE, PF>), dim3((unsigned)nb),     \
                                              dim3(z.rw * z.rh), ldsz, s, mkgrid(dims, roi),     \
                                              vold, vnew, (const S*)yold, (S*)ynew, (const S*)u, \
                                              (const S*)ut, z.rw, z.rh, z.zc, sigma1, lambda,    \
                                              ctl, iter))
-  if (pre) WAP_ZM2(true);
-  else WAP_ZM2(false);
+  if (pf) WAP_ZM2(false, true);
+  else if (pre) WAP_ZM2(true, false);
+  else WAP_ZM2(false, false);
 #undef WAP_ZM2
   return hipGetLastError();
 }
