@@ -46,6 +46,9 @@ int wap_version(void);
  *   size and keeps the fastest; 0 = two kernels (primal over the ROI, then dual over the
  *   grid); 1..5 = one kernel with LDS plane tiles; 6..29 = one single-pass kernel whose
  *   threads march z over column tiles (tile and chunk shapes listed in flow.hip).
+ *   "flow_pair": two-iterations-per-pass form of the normal-mode iterations: -1
+ *   (default) = autotuned per grid size (or off when single iterations are faster),
+ *   0 = off, k = tile shape k (1..flow_pair_count).
  *   "flow_retune" (set, any value): forget the autotuner's choices.
  *   "flow_storage": precision form of the flow solver's state: 0 = float (default; the
  *   reference's floats bit for bit), 1 = bfloat16, 2 = half storage of the duals y, the
@@ -58,7 +61,8 @@ int wap_version(void);
  *   rays per wave each walking 64/n consecutive samples on 64/n lanes.
  *   "project_unroll": ray samples per trip of the forward projector's march (1, 2, 3,
  *   4, 6, 8; default 2).
- *   get-only: "flow_variant_used" (form of the last flow level), "flow_variant_count".
+ *   get-only: "flow_variant_used" (form of the last flow level), "flow_variant_count",
+ *   "flow_pair_used" (pair shape of the last flow level, 0 = none), "flow_pair_count".
  * The autotuner synchronises the stream once per new grid size; callers that capture
  * a graph run one warm-up call first or set a fixed form. */
 int wap_set_tuning(WapContext* ctx, const char* key, int value);
