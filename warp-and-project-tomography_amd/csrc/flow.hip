@@ -504,7 +504,9 @@ __global__ __launch_bounds__(1024) void k_flow_zm(
     if (!YLDS || ylast)
       pd_point<false, S>(G, vo, yo, u, n, i, jy, kn, tn + dyb, roi_y && rz(kn), c3y, ny1);
     const float utt = lds(ut, t);
-    const float k4p = ldb(kx4prev, t);
+    // the previous u.v is read only in the first iteration of a later warp (mode is
+    // uniform)
+    const float k4p = mode == 1 ? ldb(kx4prev, t) : 0.0f;
     const float ktyo[3] = {KTY && resid ? ldb(kty, t) : 0.0f,
                            KTY && resid ? ldb(kty + n, t) : 0.0f,
                            KTY && resid ? ldb(kty + 2 * n, t) : 0.0f};

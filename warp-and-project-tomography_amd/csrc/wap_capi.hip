@@ -1044,8 +1044,35 @@ int wap_iteration(WapContext* c, const WapLoopConfig* L, float* vols, float* flo
       return fail(c, WAP_E_ARG, "wap_iteration: view window outside the scan");
   const size_t n = nvox(L->dims), nd = (size_t)L->nu * L->nv;
   hipStream_t st = S(stream);
-  for (int f = 0; f < L->nstates; ++f)
-    WAP_TRY(sart_prox_impl(c, L, vols + f * n, sino, ws[f], we[f], mask, st));
+  // The states are independent in the SART sweeps and in the warp pass (each reads the
+  // shared sinogram, views, mask and flows and writes only its own volume): they run on
+  // up to flow_streams worker streams, each with its own scratch, ordered after the work
+  // queued before on the caller's stream and before the work queued after.
+  auto fan_out = [&](auto&& per_state) -> int {
+    const int K = std::max(1, std::min(c->flow_streams, L->nstates));
+    if (K == 1) {
+      for (int f = 0; f < L->nstates; ++f) WAP_TRY(per_state(f, st));
+      return WAP_OK;
+    }
+    WAP_TRY(ensure_workers(c, K));
+    WAP_HIP(c, hipEventRecord(c->worker_events[K], st));
+    for (int k = 0; k < K; ++k)
+      WAP_HIP(c, hipStreamWaitEvent(c->workers[k], c->worker_events[K], 0));
+    int rc = WAP_OK;
+    for (int f = 0; f < L->nstates && rc == WAP_OK; ++f) {
+      c->lane = 1 + f % K;
+      rc = per_state(f, c->workers[f % K]);
+    }
+    c->lane = 0;
+    for (int k = 0; k < K; ++k) {
+      WAP_HIP(c, hipEventRecord(c->worker_events[k], c->workers[k]));
+      WAP_HIP(c, hipStreamWaitEvent(st, c->worker_events[k], 0));
+    }
+    return rc;
+  };
+  WAP_TRY(fan_out([&](int f, hipStream_t s) {
+    return sart_prox_impl(c, L, vols + f * n, sino, ws[f], we[f], mask, s);
+  }));
   if (L->do_flow) {
     // the pairs' flows are independent: they run on up to flow_streams worker streams,
     // each with its own scratch, so one pair's small pyramid levels overlap another's
@@ -1089,12 +1116,14 @@ int wap_iteration(WapContext* c, const WapLoopConfig* L, float* vols, float* flo
     }
   }
   if (L->do_warp) {
-    for (int f = 0; f < L->nstates; ++f)
+    WAP_TRY(fan_out([&](int f, hipStream_t s) {
       for (int ik = ws[f]; ik <= we[f]; ++ik) {
         float td = ((float)ik - ftime[f]) / L->timeunit;
         td = std::min(std::max(-1.0f, td), 1.0f);
-        WAP_TRY(warp_project_impl(c, L, vols, flows, sino + ik * nd, f, ik, td, mask, st));
+        WAP_TRY(warp_project_impl(c, L, vols, flows, sino + ik * nd, f, ik, td, mask, s));
       }
+      return WAP_OK;
+    }));
   }
   return WAP_OK;
 }
