@@ -592,43 +592,63 @@ int flow_level(WapContext* c, const int* dims, const float* I1, const float* I2,
   return WAP_OK;
 }
 
-int flow_multiscale_impl(WapContext* c, const int* dims, const float* I1, const float* I2,
-                         const float* V_init, float tol, float lambda, int max_iter, int nwarps,
-                         float eta, float sigma, int scales, const int* cut, float* V_out,
-                         hipStream_t st) {
-  const size_t n0 = nvox(dims);
+// Levels of the multiscale flow pyramid, coarsest first: extents (int)(dim * eta^l) and
+// the offsets of the levels in one buffer of `total` floats.
+struct PyrLevels {
   struct Lv {
     int d[3];
     size_t n, off;
   };
-  std::vector<Lv> lv(scales);
-  float t_eta = 1.0f;
+  std::vector<Lv> lv;
   size_t total = 0;
+};
+
+int pyr_levels(WapContext* c, const int* dims, float eta, int scales, PyrLevels* P) {
+  P->lv.assign(scales, PyrLevels::Lv());
+  float t_eta = 1.0f;
   for (int s = 0; s < scales; ++s) {
-    Lv& L = lv[scales - 1 - s];
+    PyrLevels::Lv& L = P->lv[scales - 1 - s];
     for (int a = 0; a < 3; ++a) L.d[a] = (int)((float)dims[a] * t_eta);
     if (L.d[0] < 4 || L.d[1] < 4 || L.d[2] < 4)
       return fail(c, WAP_E_ARG, "flow pyramid level smaller than 4 voxels; use fewer scales");
     L.n = nvox(L.d);
     t_eta = t_eta * eta;
   }
+  P->total = 0;
   for (int s = 0; s < scales; ++s) {
-    lv[s].off = total;
-    total += lv[s].n;
+    P->lv[s].off = P->total;
+    P->total += P->lv[s].n;
   }
-  float *b1, *b2, *p1, *p2, *pv, *py;
-  WAP_TRY(scratch(c, "ms_b1", n0, &b1));
-  WAP_TRY(scratch(c, "ms_b2", n0, &b2));
-  WAP_TRY(scratch(c, "ms_i1", total, &p1));
-  WAP_TRY(scratch(c, "ms_i2", total, &p2));
-  WAP_TRY(scratch(c, "ms_v", 3 * total, &pv));
-  WAP_TRY(scratch(c, "ms_y", 10 * total, &py));
-  WAP_TRY(blur_impl(c, I1, dims, 1, sigma, b1, st));
-  WAP_TRY(blur_impl(c, I2, dims, 1, sigma, b2, st));
-  for (int s = 0; s < scales; ++s) {
-    WAP_TRY(resize_impl(c, b1, dims, 1, lv[s].d, p1 + lv[s].off, st));
-    WAP_TRY(resize_impl(c, b2, dims, 1, lv[s].d, p2 + lv[s].off, st));
+  return WAP_OK;
+}
+
+// The blurred pyramid of one state (Project/optical_flow.h:1010-1047): Deriche blur at
+// full size, then every level resized from it (the full-size level is the blur itself).
+int build_pyramid(WapContext* c, const int* dims, const float* I, float sigma,
+                  const PyrLevels& P, float* pyr, hipStream_t st) {
+  float* b;
+  WAP_TRY(scratch(c, "ms_blur", nvox(dims), &b));
+  WAP_TRY(blur_impl(c, I, dims, 1, sigma, b, st));
+  for (const PyrLevels::Lv& L : P.lv) {
+    if (L.d[0] == dims[0] && L.d[1] == dims[1] && L.d[2] == dims[2])
+      WAP_HIP(c, wapk_copy(pyr + L.off, b, L.n, st));
+    else
+      WAP_TRY(resize_impl(c, b, dims, 1, L.d, pyr + L.off, st));
   }
+  return WAP_OK;
+}
+
+// The multiscale flow from the two states' pyramids (Project/optical_flow.h:1002-1126).
+int flow_multiscale_pyr(WapContext* c, const int* dims, const PyrLevels& P, const float* p1,
+                        const float* p2, const float* V_init, float tol, float lambda,
+                        int max_iter, int nwarps, const int* cut, float* V_out,
+                        hipStream_t st) {
+  const size_t n0 = nvox(dims);
+  const std::vector<PyrLevels::Lv>& lv = P.lv;
+  const int scales = (int)lv.size();
+  float *pv, *py;
+  WAP_TRY(scratch(c, "ms_v", 3 * P.total, &pv));
+  WAP_TRY(scratch(c, "ms_y", 10 * P.total, &py));
   float* v0 = pv + 3 * lv[0].off;
   if (V_init)
     WAP_TRY(resize_impl(c, V_init, dims, 3, lv[0].d, v0, st));
@@ -636,7 +656,7 @@ int flow_multiscale_impl(WapContext* c, const int* dims, const float* I1, const 
     WAP_HIP(c, wapk_fill(v0, 3 * lv[0].n, 0.0f, st));
   WAP_HIP(c, wapk_fill(py + 10 * lv[0].off, 10 * lv[0].n, 0.0f, st));
   for (int s = 0; s < scales; ++s) {
-    const Lv& L = lv[s];
+    const PyrLevels::Lv& L = lv[s];
     const float aabb[6] = {0.0f, (float)dims[0], 0.0f, (float)dims[1], 0.0f, (float)dims[2]};
     int roi[6];
     for (int q = 0; q < 6; ++q) {
@@ -648,13 +668,28 @@ int flow_multiscale_impl(WapContext* c, const int* dims, const float* I1, const 
     WAP_TRY(flow_level(c, L.d, p1 + L.off, p2 + L.off, v, y, tol, lambda, max_iter, nwarps, aabb,
                        roi, nullptr, st));
     if (s + 1 < scales) {
-      const Lv& N = lv[s + 1];
+      const PyrLevels::Lv& N = lv[s + 1];
       WAP_TRY(resize_impl(c, v, L.d, 3, N.d, pv + 3 * N.off, st));
       WAP_TRY(resize_impl(c, y, L.d, 10, N.d, py + 10 * N.off, st));
     }
   }
   WAP_HIP(c, wapk_copy(V_out, pv + 3 * lv[scales - 1].off, 3 * n0, st));
   return WAP_OK;
+}
+
+int flow_multiscale_impl(WapContext* c, const int* dims, const float* I1, const float* I2,
+                         const float* V_init, float tol, float lambda, int max_iter, int nwarps,
+                         float eta, float sigma, int scales, const int* cut, float* V_out,
+                         hipStream_t st) {
+  PyrLevels P;
+  WAP_TRY(pyr_levels(c, dims, eta, scales, &P));
+  float *p1, *p2;
+  WAP_TRY(scratch(c, "ms_i1", P.total, &p1));
+  WAP_TRY(scratch(c, "ms_i2", P.total, &p2));
+  WAP_TRY(build_pyramid(c, dims, I1, sigma, P, p1, st));
+  WAP_TRY(build_pyramid(c, dims, I2, sigma, P, p2, st));
+  return flow_multiscale_pyr(c, dims, P, p1, p2, V_init, tol, lambda, max_iter, nwarps, cut,
+                             V_out, st);
 }
 
 int check_views(WapContext* c, int view0, int nviews) {
@@ -1074,6 +1109,14 @@ int wap_iteration(WapContext* c, const WapLoopConfig* L, float* vols, float* flo
     return sart_prox_impl(c, L, vols + f * n, sino, ws[f], we[f], mask, s);
   }));
   if (L->do_flow) {
+    // every state's blurred pyramid once (each serves the pairs on both sides of it)
+    PyrLevels P;
+    WAP_TRY(pyr_levels(c, L->dims, L->flow_eta, L->flow_scales, &P));
+    float* pyr;
+    WAP_TRY(scratch(c, "iter_pyr", (size_t)L->nstates * P.total, &pyr));
+    WAP_TRY(fan_out([&](int f, hipStream_t s) {
+      return build_pyramid(c, L->dims, vols + f * n, L->flow_sigma, P, pyr + f * P.total, s);
+    }));
     // the pairs' flows are independent: they run on up to flow_streams worker streams,
     // each with its own scratch, so one pair's small pyramid levels overlap another's
     // full-size level
@@ -1082,10 +1125,9 @@ int wap_iteration(WapContext* c, const WapLoopConfig* L, float* vols, float* flo
       float* out;
       WAP_TRY(scratch(c, "iter_flow", 3 * n, &out));
       for (int f = 0; f + 1 < L->nstates; ++f) {
-        WAP_TRY(flow_multiscale_impl(c, L->dims, vols + f * n, vols + (f + 1) * n,
-                                     flows + f * 3 * n, L->flow_tol, L->flow_lambda,
-                                     L->flow_iters, L->flow_warps, L->flow_eta, L->flow_sigma,
-                                     L->flow_scales, L->cut, out, st));
+        WAP_TRY(flow_multiscale_pyr(c, L->dims, P, pyr + f * P.total, pyr + (f + 1) * P.total,
+                                    flows + f * 3 * n, L->flow_tol, L->flow_lambda,
+                                    L->flow_iters, L->flow_warps, L->cut, out, st));
         WAP_HIP(c, wapk_copy(flows + f * 3 * n, out, 3 * n, st));
       }
     } else {
@@ -1100,10 +1142,9 @@ int wap_iteration(WapContext* c, const WapLoopConfig* L, float* vols, float* flo
         float* out;
         rc = scratch(c, "iter_flow", 3 * n, &out);
         if (rc == WAP_OK)
-          rc = flow_multiscale_impl(c, L->dims, vols + f * n, vols + (f + 1) * n,
-                                    flows + f * 3 * n, L->flow_tol, L->flow_lambda,
-                                    L->flow_iters, L->flow_warps, L->flow_eta, L->flow_sigma,
-                                    L->flow_scales, L->cut, out, ws_);
+          rc = flow_multiscale_pyr(c, L->dims, P, pyr + f * P.total, pyr + (f + 1) * P.total,
+                                   flows + f * 3 * n, L->flow_tol, L->flow_lambda,
+                                   L->flow_iters, L->flow_warps, L->cut, out, ws_);
         if (rc == WAP_OK && wapk_copy(flows + f * 3 * n, out, 3 * n, ws_) != hipSuccess)
           rc = fail(c, WAP_E_HIP, "wap_iteration: flow copy failed");
       }
