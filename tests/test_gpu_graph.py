@@ -1,0 +1,132 @@
+"""A whole WaP outer iteration captured into a HIP graph and replayed.
+
+After one warm-up call has settled the autotuners' choices and the scratch sizes,
+wap_iteration queues only kernels and event records/waits: the worker streams fork from
+the caller's stream through events and join back the same way, and fills are kernels
+(no memset nodes). So the call can be captured with torch.cuda.graph and replayed. Each
+replay here starts from the same state, with eager work and synchronisations in between,
+and must give the eager iteration's floats bit for bit.
+"""
+import numpy as np
+import pytest
+
+from test_gpu_parity import DEV, H, T, _loop_setup
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("streams", [1, 4])
+def test_iteration_graph_replays_match_eager(wctx, oracle, W, streams):
+    prev = wctx.get_tuning("flow_streams")
+    wctx.set_tuning("flow_streams", streams)
+    try:
+        shape, cfg, ocfg, views, oviews, sino, ws, we, ft, mask = _loop_setup(W, oracle, False)
+        wctx.upload_views(views)
+        n = cfg.nstates
+        sino_t, mask_t = T(sino), T(mask)
+        vols = torch.zeros((n,) + shape, device=DEV)
+        flows = torch.zeros((n, 3) + shape, device=DEV)
+        side = torch.cuda.Stream()
+        side.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(side):
+            # eager reference (and the warm-up: autotuning, scratch, tables)
+            wctx.iteration(cfg, vols, flows, sino_t, ws, we, ft, mask_t)
+            torch.cuda.synchronize()
+            ref_v, ref_f = H(vols), H(flows)
+            assert ref_v.max() > 0.1 and np.abs(ref_f).max() > 0.0
+            vols.zero_()
+            flows.zero_()
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            wctx.iteration(cfg, vols, flows, sino_t, ws, we, ft, mask_t)
+        torch.cuda.synchronize()
+        for rep in range(3):
+            vols.zero_()
+            flows.zero_()
+            torch.cuda.synchronize()
+            print(f"graph replay {rep} with {streams} worker streams", flush=True)
+            g.replay()
+            torch.cuda.synchronize()
+            np.testing.assert_array_equal(H(vols), ref_v)
+            np.testing.assert_array_equal(H(flows), ref_f)
+            # eager work between replays: a second iteration on top, compared with eager
+            wctx.iteration(cfg, vols, flows, sino_t, ws, we, ft, mask_t)
+            torch.cuda.synchronize()
+            assert np.isfinite(H(vols)).all()
+        del g
+    finally:
+        wctx.set_tuning("flow_streams", prev)
+
+
+def _small_loop(W, oracle, shape, nstates, per, nu, seed):
+    """A small cone-beam reconstruction problem of its own size (helpers' generators)."""
+    import pyoracle
+
+    import helpers
+    from helpers import SD
+    nviews = nstates * per
+    geo = helpers.cone_geometry(W, shape, nu, nu)
+    views = W.views(geo, helpers.phantoms.golden_angles(nviews, 360.0))
+    base = helpers.blob_volume(shape, seed)
+    flow = helpers.smooth_flow(shape, seed, 1.0)
+    truth = np.stack([oracle.advect(flow, base, float(s) / nstates) for s in range(nstates)])
+    oviews = (pyoracle.WapView * nviews).from_buffer_copy(bytes(views))
+    sino = np.stack([oracle.forward_project(truth[i // per], oviews[i], nu, nu, SD)[0]
+                     for i in range(nviews)]).astype(np.float32)
+    cfg = W.WapLoopConfig()
+    cfg.dims[:] = [shape[2], shape[1], shape[0]]
+    cfg.nstates, cfg.nviews, cfg.nu, cfg.nv = nstates, nviews, nu, nu
+    cfg.sd, cfg.alpha, cfg.gmax, cfg.nb_sart = SD, 0.2, float(sino.max()), 2
+    cfg.roi[:] = [2, 2, 2, 2, 2, 2, 3, 3]
+    cfg.cut[:] = [2] * 8
+    cfg.flow_tol, cfg.flow_lambda, cfg.flow_eta, cfg.flow_sigma = 0.008, 0.05, 0.5, 0.8
+    cfg.flow_iters, cfg.flow_warps, cfg.flow_scales = 12, 2, 2
+    cfg.do_flow, cfg.do_warp, cfg.timeunit = 1, 1, float(per)
+    ws = [per * s for s in range(nstates)]
+    we = [w + per - 1 for w in ws]
+    ft = [per * s + 1.5 for s in range(nstates)]
+    mask = oracle.roi_mask(shape, list(cfg.roi))
+    return cfg, views, sino, ws, we, ft, mask
+
+
+def test_uneven_reconstructions_run_concurrently(oracle, W):
+    """Independent reconstructions of uneven sizes, each with its own context and stream,
+    queued together, give what each gives run alone (contexts share nothing)."""
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    problems = [((20, 20, 20), 3, 4, 30, 31), ((28, 20, 24), 4, 3, 36, 32),
+                ((16, 18, 22), 2, 5, 26, 33), ((24, 24, 24), 3, 4, 34, 34)]
+    setups = [_small_loop(W, oracle, *p) for p in problems]
+
+    def run(concurrent):
+        ctxs, outs, streams = [], [], []
+        for (shape, nstates, *_), (cfg, views, sino, ws, we, ft, mask) in zip(problems, setups):
+            ctx = W.Context(0)
+            ctx.upload_views(views)
+            s = torch.cuda.Stream() if concurrent else torch.cuda.current_stream()
+            vols = torch.zeros((nstates,) + shape, device=DEV)
+            flows = torch.zeros((nstates, 3) + shape, device=DEV)
+            sino_t, mask_t = T(sino), T(mask)
+            s.wait_stream(torch.cuda.current_stream())
+            with torch.cuda.stream(s):
+                for _ in range(2):
+                    ctx.iteration(cfg, vols, flows, sino_t, ws, we, ft, mask_t)
+            if not concurrent:
+                torch.cuda.synchronize()
+            ctxs.append(ctx)
+            outs.append((vols, flows, sino_t, mask_t))
+            streams.append(s)
+        torch.cuda.synchronize()
+        res = [(H(v), H(f)) for v, f, _, _ in outs]
+        for ctx in ctxs:
+            ctx.close()
+        return res
+
+    alone = run(False)
+    together = run(True)
+    for (va, fa), (vt, ft_) in zip(alone, together):
+        np.testing.assert_array_equal(vt, va)
+        np.testing.assert_array_equal(ft_, fa)
+        assert va.max() > 0.05
