@@ -13,7 +13,9 @@ warp-and-project pass over all views.
 Prints one JSON line with the metric, the projector throughput (GRays/s over a full
 192-view sweep), the roofline of the dominant kernel (the flow primal-dual iteration,
 timed live with events on the stream it runs on) and the oracle's CPU time for the same
-work extrapolated from a bounded sample of kernel units.
+work extrapolated from a bounded sample of kernel units. Unless --no-forms is given, the
+line also carries the precision form (the same step with the flow state stored in half)
+and the batching form (8 uneven reconstructions, one after another and concurrently).
 """
 import argparse
 import json
@@ -53,7 +55,8 @@ def parse():
                         "state at full size projected along 8 views, then the same calibration "
                         "copy as --profile-pd (use under rocprofv3)")
     p.add_argument("--no-forms", action="store_true",
-                   help="skip the precision-form step (half storage of the flow state)")
+                   help="skip the precision-form step (half storage of the flow state) and the "
+                        "batching-form sample (8 uneven reconstructions)")
     return p.parse_args()
 
 
@@ -241,6 +244,120 @@ def projector_rate(ctx, torch, vols, sd, S, NDET, L):
             "hbm_gbs_algorithmic": round(gbs, 1), "hbm_frac": round(gbs / HBM_PEAK_GBS, 4)}
 
 
+def batching_form(W, torch, dev):
+    """The batching form on a bounded sample: 8 independent reconstructions of uneven
+    sizes (96^3..208^3, 4..12 states, 2 views per state, detector 1.5L, blob phantoms
+    moving over the states), one WaP outer iteration each (5 SART sweeps, 3-scale flow),
+    each with its own context. Timed run one after another and queued together on one
+    stream each. Returns the form's JSON object."""
+    import numpy as np
+
+    import phantoms
+    rng = phantoms.SplitMix64(phantoms.SEED_BASE + 4)
+    sizes = [96, 112, 128, 144, 160, 176, 192, 208]
+    recs = []
+    for L in sizes:
+        S = 4 + int(rng.uniform(0.0, 9.0))
+        per = 2
+        V = S * per
+        NDET = L * 3 // 2
+        ctx = W.Context(0)
+        geo = W.make_geometry((L, L, L), NDET, NDET, ds=1.0, sid=4.0 * L, sdd=6.0 * L)
+        ctx.upload_views(W.views(geo, phantoms.golden_angles(V, 360.0)))
+        lin = torch.arange(L, device=dev, dtype=torch.float32)
+        gz, gy, gx = torch.meshgrid(lin, lin, lin, indexing="ij")
+        truth = torch.zeros((S, L, L, L), device=dev)
+        for _ in range(20):
+            c = [rng.uniform(0.3 * L, 0.7 * L) for _ in range(3)]
+            d = [rng.uniform(-0.05 * L, 0.05 * L) for _ in range(3)]
+            sig = rng.uniform(0.03 * L, 0.08 * L)
+            amp = rng.uniform(0.2, 1.0)
+            for st in range(S):
+                f = st / max(S - 1, 1)
+                r2 = ((gx - c[0] - f * d[0]) ** 2 + (gy - c[1] - f * d[1]) ** 2 +
+                      (gz - c[2] - f * d[2]) ** 2)
+                truth[st] += amp * torch.exp(-r2 / (2.0 * sig * sig))
+        del gz, gy, gx
+        sd = float(np.float32(math.sqrt(3.0) / 2.0))
+        sino = torch.zeros((V, NDET, NDET), device=dev)
+        for st in range(S):
+            ctx.forward_project(truth[st], sd, per * st, per, NDET, NDET,
+                                sino[per * st:per * st + per])
+        cfg = W.WapLoopConfig()
+        cfg.dims[:] = [L, L, L]
+        cfg.nstates, cfg.nviews, cfg.nu, cfg.nv = S, V, NDET, NDET
+        cfg.sd, cfg.alpha, cfg.gmax, cfg.nb_sart = sd, 0.1, float(sino[0].max()), 5
+        roi = max(1, round(8 * L / 256))
+        cfg.roi[:] = [roi] * 8
+        cfg.cut[:] = [roi] * 8
+        cfg.flow_tol, cfg.flow_lambda, cfg.flow_eta, cfg.flow_sigma = 0.008, 0.05, 0.5, 0.8
+        cfg.flow_iters, cfg.flow_warps, cfg.flow_scales = 50, 3, 3
+        cfg.do_flow, cfg.do_warp, cfg.timeunit = 1, 1, float(per)
+        ws = [per * st for st in range(S)]
+        we = [w + per - 1 for w in ws]
+        ft = [per * st + 0.5 for st in range(S)]
+        mask = torch.from_numpy(_roi_mask(W, L, roi)).to(dev)
+        vols = torch.zeros((S, L, L, L), device=dev)
+        flows = torch.zeros((S, 3, L, L, L), device=dev)
+        recs.append(dict(ctx=ctx, cfg=cfg, sino=sino, ws=ws, we=we, ft=ft, mask=mask,
+                         vols=vols, flows=flows, L=L, S=S))
+        del truth
+        torch.cuda.synchronize()
+        print(f"[bench] batching form: {L}^3 x {S} states set up", file=sys.stderr, flush=True)
+    streams = [torch.cuda.Stream() for _ in recs]
+
+    def reset():
+        for r in recs:
+            r["vols"].zero_()
+            r["flows"].zero_()
+
+    def one(r):
+        r["ctx"].iteration(r["cfg"], r["vols"], r["flows"], r["sino"], r["ws"], r["we"],
+                           r["ft"], r["mask"])
+
+    def timed(fn):
+        reset()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) * 1000.0
+
+    def sequential(verbose=False):
+        for r in recs:
+            one(r)
+            if verbose:
+                torch.cuda.synchronize()
+                print(f"[bench] batching form: warm-up of {r['L']}^3 x {r['S']} done",
+                      file=sys.stderr, flush=True)
+
+    def concurrent():
+        cur = torch.cuda.current_stream()
+        for st, r in zip(streams, recs):
+            st.wait_stream(cur)
+            with torch.cuda.stream(st):
+                one(r)
+        for st in streams:
+            cur.wait_stream(st)
+
+    timed(lambda: sequential(True))  # warm-up: autotuning of every size, scratch
+    print("[bench] batching form: warm-up done", file=sys.stderr, flush=True)
+    seq = timed(sequential)
+    alone = [r["vols"].clone() for r in recs]
+    print("[bench] batching form: sequential run done", file=sys.stderr, flush=True)
+    con = timed(concurrent)
+    same = all(torch.equal(r["vols"], a) for r, a in zip(recs, alone))
+    print("[bench] batching form: concurrent run done", file=sys.stderr, flush=True)
+    del alone
+    for r in recs:
+        r["ctx"].close()
+    return {"reconstructions": len(recs), "sizes": sizes, "states": [r["S"] for r in recs],
+            "sample": "8 uneven reconstructions x 1 outer iteration (5 SART sweeps, 3-scale "
+                      "flow, 2 views per state, detector 1.5L)",
+            "sequential_ms": round(seq, 1), "concurrent_ms": round(con, 1),
+            "speedup_concurrent": round(seq / con, 3), "concurrent_matches_sequential": same}
+
+
 def cpu_baseline(L, S, NDET, sart, flow_iters, budget_s):
     """The oracle (our OpenMP CPU restatement) timed on this host on single kernel units
     at full size, extrapolated to one outer iteration with the same unit counts."""
@@ -399,6 +516,8 @@ def main():
                                "error_vs_fp64": "flow <= 5e-4, volumes <= 1e-4 rel L2 "
                                                 "(tests/test_gpu_precision.py)"}}
         print("[bench] precision-form step done", file=sys.stderr, flush=True)
+        forms["batching"] = batching_form(W, torch, dev)
+        print("[bench] batching-form sample done", file=sys.stderr, flush=True)
     del flows, vols
     torch.cuda.empty_cache()
     roof = roofline_pd(ctx, torch, dev, L)
