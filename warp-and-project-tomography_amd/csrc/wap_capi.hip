@@ -692,6 +692,42 @@ int flow_multiscale_impl(WapContext* c, const int* dims, const float* I1, const 
                              V_out, st);
 }
 
+// Runs the flow autotuners for every level size of the pyramid that has no choice yet,
+// on one stream before any pair is queued, so that their timings are not disturbed by
+// other pairs' kernels on the worker streams. A level with no warps runs the tuners on
+// the buffers of the scratch set `lane` will use (no extra memory) and iterates nothing.
+int pretune_levels(WapContext* c, const WapLoopConfig* L, const PyrLevels& P, const float* pyr,
+                   int lane, hipStream_t st) {
+  const int saved = c->lane;
+  c->lane = lane;
+  int rc = WAP_OK;
+  for (const PyrLevels::Lv& lv : P.lv) {
+    const std::array<int, 4> key = {lv.d[0], lv.d[1], lv.d[2], c->flow_storage};
+    if ((c->flow_variant >= 0 || c->flow_tuned.count(key)) &&
+        (c->flow_pair >= 0 || c->pair_tuned.count(key)))
+      continue;
+    float *pv, *py;
+    rc = scratch(c, "ms_v", 3 * P.total, &pv);
+    if (rc == WAP_OK) rc = scratch(c, "ms_y", 10 * P.total, &py);
+    if (rc != WAP_OK) break;
+    int roi[6];
+    for (int q = 0; q < 6; ++q)
+      roi[q] = (int)((float)L->cut[q] * ((float)lv.d[q / 2] / (float)L->dims[q / 2]));
+    const float aabb[6] = {0.0f, (float)L->dims[0], 0.0f, (float)L->dims[1], 0.0f,
+                           (float)L->dims[2]};
+    if (wapk_fill(pv, 3 * lv.n, 0.0f, st) != hipSuccess ||
+        wapk_fill(py, 10 * lv.n, 0.0f, st) != hipSuccess) {
+      rc = fail(c, WAP_E_HIP, "flow level tuning: fill failed");
+      break;
+    }
+    rc = flow_level(c, lv.d, pyr + lv.off, pyr + lv.off, pv, py, L->flow_tol, L->flow_lambda,
+                    0, 0, aabb, roi, nullptr, st);
+    if (rc != WAP_OK) break;
+  }
+  c->lane = saved;
+  return rc;
+}
+
 int check_views(WapContext* c, int view0, int nviews) {
   if (!c->views || view0 < 0 || nviews <= 0 || view0 + nviews > c->nviews)
     return fail(c, WAP_E_STATE, "view range outside the uploaded view table");
@@ -1121,6 +1157,7 @@ int wap_iteration(WapContext* c, const WapLoopConfig* L, float* vols, float* flo
     // each with its own scratch, so one pair's small pyramid levels overlap another's
     // full-size level
     const int K = std::max(1, std::min(c->flow_streams, L->nstates - 1));
+    WAP_TRY(pretune_levels(c, L, P, pyr, K > 1 ? 1 : 0, st));
     if (K == 1) {
       float* out;
       WAP_TRY(scratch(c, "iter_flow", 3 * n, &out));
