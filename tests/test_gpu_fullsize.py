@@ -1,0 +1,207 @@
+"""The kernels at the benchmark's full sizes against the oracle, and properties that
+hold at any size, checked at those sizes.
+
+Full size: a 256^3 volume, a 384^2 detector and the benchmark's cone beam (source at 4L,
+detector at 6L). The oracle finishes these cases in seconds on the host's cores:
+- forward projection of four views, SART correction and voxel backprojection of one view;
+- advection of a 256^3 state;
+- a flow solve of 7 iterations x 2 warps, which runs single iterations and pair passes of
+  the shape autotuned for 256^3;
+- the SART sweeps and the warp-and-project pass of a 4-state outer iteration.
+All of them must agree bit for bit. The size-independent properties:
+- equal images and a zero start keep the flow and its TV duals at zero (up to subnormals
+  where the gradient is subnormal), and the residual test stops the solve at iteration 50;
+- a zero flow advects every voxel off the boundary layer to itself;
+- <A x, y> = <x, A^T y> for the ray projector and its matched backprojector, in FP64;
+- a unit volume projects to sd times the sample count of each ray.
+"""
+import numpy as np
+import pytest
+
+import helpers
+import phantoms
+from helpers import SD
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+DEV = "cuda:0"
+L, NU = 256, 384
+SID, SDD = 4.0 * L, 6.0 * L
+ANGLES = [0.0, 90.0, 37.5, 222.49223594996215]
+
+
+def T(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).to(DEV)
+
+
+def H(t):
+    torch.cuda.synchronize()
+    return t.detach().cpu().numpy()
+
+
+@pytest.fixture(scope="module")
+def foam():
+    """The benchmark's base phantom: a unit-density cube with 400 ellipsoidal voids."""
+    return phantoms.foam_phantom(L, 400, phantoms.SplitMix64(phantoms.SEED_BASE + 2))
+
+
+@pytest.fixture(scope="module")
+def scan(W):
+    geo = W.make_geometry((L, L, L), NU, NU, ds=1.0, sid=SID, sdd=SDD)
+    return W.views(geo, ANGLES)
+
+
+def test_fullsize_forward_projection(wctx, oracle, foam, scan):
+    wctx.upload_views(scan)
+    n = len(ANGLES)
+    proj = torch.zeros((n, NU, NU), device=DEV)
+    rl = torch.zeros_like(proj)
+    wt = torch.zeros_like(proj)
+    wctx.forward_project(T(foam), SD, 0, n, NU, NU, proj, rl, wt)
+    P, R, Wt = H(proj), H(rl), H(wt)
+    for k in range(n):
+        print(f"full-size view {k} ({ANGLES[k]} degrees)", flush=True)
+        a, b, c = oracle.forward_project(foam, scan[k], NU, NU, SD)
+        np.testing.assert_array_equal(Wt[k], c)
+        np.testing.assert_array_equal(R[k], b)
+        np.testing.assert_array_equal(P[k], a)
+    assert (Wt > 200).mean() > 0.3  # long chords through the cube
+
+
+def test_fullsize_sart_view(wctx, oracle, foam, scan):
+    wctx.upload_views(scan)
+    rng = np.random.default_rng(21)
+    b = (rng.random((NU, NU)) * 100).astype(np.float32)
+    py0 = (rng.standard_normal((NU, NU)) * 0.1).astype(np.float32)
+    corr = torch.zeros((NU, NU), device=DEV)
+    py = T(py0)
+    wctx.project_correct(T(foam), SD, 2, NU, NU, T(b), py, 0.1, corr)
+    ax, len_, _ = oracle.forward_project(foam, scan[2], NU, NU, SD)
+    c_ref, py_ref = oracle.sart_correct(b, ax, len_, py0, 0.1)
+    np.testing.assert_array_equal(H(corr), c_ref)
+    np.testing.assert_array_equal(H(py), py_ref)
+    lo, hi = (8, 8, 8), (L - 8, L - 8, L - 8)
+    v = T(foam)
+    wctx.backproject_voxel(v, corr, 2, 0.1, 1.0, lo, hi, 0)
+    ref = oracle.backproject_voxel(foam, c_ref, scan[2], 0.1, 1.0, lo, hi, 0)
+    np.testing.assert_array_equal(H(v), ref)
+    assert np.abs(ref - foam).max() > 0
+
+
+def test_fullsize_advect(wctx, oracle, foam):
+    shape = (L, L, L)
+    vel = helpers.smooth_flow(shape, 21, 2.0)
+    out = torch.zeros(shape, device=DEV)
+    wctx.advect(T(vel), T(foam), -1.0, out)
+    np.testing.assert_array_equal(H(out), oracle.advect(vel, foam, -1.0))
+    # a zero flow leaves every voxel off the boundary layer in place
+    wctx.advect(torch.zeros((3,) + shape, device=DEV), T(foam), 1.0, out)
+    np.testing.assert_array_equal(H(out)[1:-1, 1:-1, 1:-1], foam[1:-1, 1:-1, 1:-1])
+
+
+@pytest.fixture(scope="module")
+def flow_images(oracle, foam):
+    I1 = oracle.blur(foam, 1.0)
+    I2 = oracle.advect(np.full((3, L, L, L), 0.4, np.float32), I1, 1.0)
+    return I1, I2
+
+
+def test_fullsize_flow_solve(wctx, oracle, flow_images):
+    I1, I2 = flow_images
+    shape = (L, L, L)
+    aabb = [0, L, 0, L, 0, L]
+    roi = [8] * 6
+    V = torch.zeros((3,) + shape, device=DEV)
+    Y = torch.zeros((10,) + shape, device=DEV)
+    it = wctx.flow_solve(T(I1), T(I2), V, Y, 0.008, 0.05, 6, 2, aabb, roi, count=True)
+    print("pair-pass shape at 256^3:", wctx.get_tuning("flow_pair_used"), flush=True)
+    z3 = np.zeros((3,) + shape, np.float32)
+    z1 = np.zeros(shape, np.float32)
+    v, y1, y2, y3, y4, it_ref = oracle.flow_single(I1, I2, z3, z3, z3, z3, z1, 0.008, 0.05,
+                                                   6, 2, aabb, roi)
+    assert it == it_ref == 14
+    np.testing.assert_array_equal(H(V), v)
+    np.testing.assert_array_equal(H(Y), np.concatenate([y1, y2, y3, y4[None]]))
+    assert np.abs(v).max() > 0.01
+
+
+def test_fullsize_flow_of_equal_images_stays_zero(wctx, flow_images):
+    I1, _ = flow_images
+    shape = (L, L, L)
+    V = torch.zeros((3,) + shape, device=DEV)
+    Y = torch.zeros((10,) + shape, device=DEV)
+    it = wctx.flow_solve(T(I1), T(I1), V, Y, 0.008, 0.05, 80, 1, [0, L, 0, L, 0, L], [8] * 6,
+                         count=True)
+    # the first residual (iteration 50) is not above the tolerance. The flow and the TV
+    # duals stay zero except where the blur's far tail leaves subnormal gradients: there
+    # sigma2 = 1/|u| overflows, y4 saturates at 1 and u*y4 moves v by a few subnormals
+    assert it == 50
+    assert np.abs(H(V)).max() < 1e-30
+    assert np.abs(H(Y)[0:9]).max() < 1e-30
+
+
+def test_fullsize_projector_adjoint_and_unit_volume(wctx, W, scan):
+    wctx.upload_views(scan)
+    shape = (L, L, L)
+    rng = np.random.default_rng(22)
+    x = rng.random(shape, dtype=np.float32)
+    y = rng.standard_normal((2, NU, NU)).astype(np.float32)
+    proj = torch.zeros((2, NU, NU), device=DEV)
+    wctx.forward_project(T(x), SD, 1, 2, NU, NU, proj)
+    at = torch.zeros(shape, device=DEV)
+    wctx.backproject_ray(T(y), shape, SD, 1, 2, float(SD), at)
+    lhs = float(np.dot(H(proj).ravel().astype(np.float64), y.ravel().astype(np.float64)))
+    rhs = float(np.dot(x.ravel().astype(np.float64), H(at).ravel().astype(np.float64)))
+    assert abs(lhs - rhs) / abs(lhs) < 1e-6
+    # a unit volume: every sample is 1 (up to the trilinear weights' rounding), so each ray
+    # integrates to sd times its sample count, the duplicated first sample included
+    wt = torch.zeros_like(proj)
+    wctx.forward_project(torch.ones(shape, device=DEV), SD, 1, 2, NU, NU, proj, None, wt)
+    P, Wt = H(proj), H(wt)
+    hit = Wt > 0
+    assert hit.mean() > 0.3
+    np.testing.assert_allclose(P[hit], float(SD) * Wt[hit], rtol=1e-4)
+    assert not P[~hit].any()
+
+
+def test_fullsize_sart_sweeps_and_warp_pass(wctx, oracle, W, foam):
+    """One outer iteration of 4 states x 2 views at full size with the flows given (the
+    flow solve itself is covered above): the SART sweep of each state and the
+    warp-and-project pass over all 8 views, against the oracle."""
+    import pyoracle
+    nstates, per = 4, 2
+    nviews = nstates * per
+    shape = (L, L, L)
+    geo = W.make_geometry((L, L, L), NU, NU, ds=1.0, sid=SID, sdd=SDD)
+    views = W.views(geo, phantoms.golden_angles(nviews, 360.0))
+    oviews = (pyoracle.WapView * nviews).from_buffer_copy(bytes(views))
+    wctx.upload_views(views)
+    sino = np.stack([oracle.forward_project(foam, oviews[i], NU, NU, SD)[0]
+                     for i in range(nviews)]).astype(np.float32)
+    cfg = W.WapLoopConfig()
+    cfg.dims[:] = [L, L, L]
+    cfg.nstates, cfg.nviews, cfg.nu, cfg.nv = nstates, nviews, NU, NU
+    cfg.sd, cfg.alpha, cfg.gmax, cfg.nb_sart = SD, 0.1, float(sino.max()), 1
+    cfg.roi[:] = [8] * 8
+    cfg.cut[:] = [8] * 8
+    cfg.flow_tol, cfg.flow_lambda, cfg.flow_eta, cfg.flow_sigma = 0.008, 0.05, 0.5, 0.8
+    cfg.flow_iters, cfg.flow_warps, cfg.flow_scales = 50, 3, 5
+    cfg.do_flow, cfg.do_warp, cfg.timeunit = 0, 1, float(per)
+    ws = np.array([per * s for s in range(nstates)], np.int32)
+    we = ws + per - 1
+    ft = np.array([per * s + 0.5 for s in range(nstates)], np.float32)
+    mask = oracle.roi_mask(shape, list(cfg.roi))
+    flows0 = np.stack([helpers.smooth_flow(shape, 30 + s, 1.5) for s in range(nstates)])
+    vols = torch.zeros((nstates,) + shape, device=DEV)
+    flows = T(flows0)
+    wctx.iteration(cfg, vols, flows, T(sino), ws, we, ft, T(mask))
+    ocfg = pyoracle.OrcLoop.from_buffer_copy(bytes(cfg))
+    rv, rf = oracle.wap_iteration(ocfg, oviews, sino, ws, we, ft,
+                                  np.zeros((nstates,) + shape, np.float32), flows0, mask)
+    np.testing.assert_array_equal(H(flows), rf)
+    got = H(vols)
+    np.testing.assert_array_equal(got, rv)
+    assert got.max() > 0.0
+    assert got.min() >= 0.0 and got.max() <= cfg.gmax
+    assert not got[:, mask == 0].any()
