@@ -193,18 +193,41 @@ WAP_DEV float ldv(const float* __restrict__ base, unsigned off) {
   return *(const float*)((const char*)base + off);
 }
 
+WAP_DEV float2 ldv2(const float* __restrict__ base, unsigned off) {
+  float2 v;
+  __builtin_memcpy(&v, (const char*)base + off, sizeof(v));
+  return v;
+}
+
 WAP_DEV float trilinear32(const float* __restrict__ vol, float p0, float p1, float p2, int W,
                           int H, int D) {
   const Corner c = clamp_pos(p0, p1, p2, W, H, D);
   const unsigned o = 4u * ((unsigned)c.i0 + (unsigned)W * ((unsigned)c.i1 + (unsigned)H *
                                                                                (unsigned)c.i2));
-  const unsigned ox = c.i0 < W - 1 ? 4u : 0u;
   const unsigned oy = c.i1 < H - 1 ? 4u * (unsigned)W : 0u;
   const unsigned oz = c.i2 < D - 1 ? 4u * (unsigned)W * (unsigned)H : 0u;
-  const float v000 = ldv(vol, o), v100 = ldv(vol, o + ox);
-  const float v010 = ldv(vol, o + oy), v110 = ldv(vol, o + oy + ox);
-  const float v001 = ldv(vol, o + oz), v101 = ldv(vol, o + oz + ox);
-  const float v011 = ldv(vol, o + oz + oy), v111 = ldv(vol, o + oz + oy + ox);
+  float v000, v100, v010, v110, v001, v101, v011, v111;
+  if (W > 1) {
+    // one 8-byte load per x pair of taps; at the upper x face (i0 = W-1) the pair starts
+    // one voxel lower, and both taps are its upper value (the clamped +x neighbour)
+    const bool edge = c.i0 >= W - 1;
+    const unsigned q = edge ? o - 4u : o;
+    const float2 a = ldv2(vol, q), b = ldv2(vol, q + oy);
+    const float2 e = ldv2(vol, q + oz), f = ldv2(vol, q + oz + oy);
+    v000 = edge ? a.y : a.x;
+    v100 = a.y;
+    v010 = edge ? b.y : b.x;
+    v110 = b.y;
+    v001 = edge ? e.y : e.x;
+    v101 = e.y;
+    v011 = edge ? f.y : f.x;
+    v111 = f.y;
+  } else {
+    v000 = v100 = ldv(vol, o);
+    v010 = v110 = ldv(vol, o + oy);
+    v001 = v101 = ldv(vol, o + oz);
+    v011 = v111 = ldv(vol, o + oz + oy);
+  }
   float g2 = c.f0, g1 = 1.0f - g2;
   float a = v000 * g1 + v100 * g2;
   float b = v010 * g1 + v110 * g2;
@@ -333,13 +356,14 @@ __global__ __launch_bounds__(256) void k_project(
     const float* __restrict__ vol, size_t vol_stride, int W, int H, int D, float sd,
     const WapView* __restrict__ views, int nu, int nv, float* __restrict__ proj,
     float* __restrict__ raylen, float* __restrict__ weight, const float* __restrict__ b,
-    float* __restrict__ py, float* __restrict__ corr, double alpha) {
+    float* __restrict__ py, float* __restrict__ corr, double alpha,
+    const int* __restrict__ perm, int pbase) {
   const unsigned tilesU = (nu + 15) / 16, tilesV = (nv + 15) / 16;
   const unsigned bid = xcd_remap(blockIdx.x, tilesU * tilesV);
   int i, j;
   pixel_of(bid, tilesU, i, j);
   if (i >= nu || j >= nv) return;
-  const int view = blockIdx.y;
+  const int view = perm ? perm[blockIdx.y] - pbase : (int)blockIdx.y;
   const WapView& v = views[view];
   const float* V = vol + vol_stride * view;
   const RaySeg r = produce_ray(v, W, H, D, i, j);
@@ -376,7 +400,8 @@ __global__ __launch_bounds__(256) void k_project_rays(
     const float* __restrict__ vol, size_t vol_stride, int W, int H, int D, float sd,
     const WapView* __restrict__ views, int nu, int nv, float* __restrict__ proj,
     float* __restrict__ raylen, float* __restrict__ weight, const float* __restrict__ b,
-    float* __restrict__ py, float* __restrict__ corr, double alpha) {
+    float* __restrict__ py, float* __restrict__ corr, double alpha,
+    const int* __restrict__ perm, int pbase) {
   constexpr int G = 64 / R;
   const int lane = (int)(threadIdx.x & 63u), wave = (int)(threadIdx.x >> 6);
   const int grp = lane / G, g = lane % G, base = grp * G;
@@ -384,7 +409,7 @@ __global__ __launch_bounds__(256) void k_project_rays(
   const int i = (int)(blockIdx.x % (unsigned)tilesU) * R + grp;
   const int j = (int)(blockIdx.x / (unsigned)tilesU) * 4 + wave;
   const bool pix_ok = i < nu && j < nv;
-  const int view = blockIdx.y;
+  const int view = perm ? perm[blockIdx.y] - pbase : (int)blockIdx.y;
   const WapView& v = views[view];
   const float* V = vol + vol_stride * view;
   // every lane of the wave runs the march (shuffles need them all); pixels past the
@@ -588,13 +613,15 @@ __global__ __launch_bounds__(256) void k_bp_voxel(float* __restrict__ vol, int W
   const int px = (int)fx, py = (int)fy;
   const float dx = fx - (float)px, dy = fy - (float)py;
   if (px < 2 || px >= nu - 2 || py < 2 || py >= nv - 2) return;
-  const float* pc = corr + (size_t)py * nu + px;
-  float arr[4];
+  // the 4x4 taps as four 16-byte row loads (px-1..px+2 lie inside the detector row)
+  const float* pc = corr + (size_t)py * nu + (px - 1);
+  float4 r[4];
 #pragma unroll
-  for (int a = 0; a < 4; ++a) {
-    const float* col = pc + (a - 1);
-    arr[a] = cubic_bp(col[-nu], col[0], col[nu], col[2 * nu], dy);
-  }
+  for (int b = 0; b < 4; ++b) __builtin_memcpy(&r[b], pc + (ptrdiff_t)(b - 1) * nu, sizeof(float4));
+  const float arr[4] = {cubic_bp(r[0].x, r[1].x, r[2].x, r[3].x, dy),
+                        cubic_bp(r[0].y, r[1].y, r[2].y, r[3].y, dy),
+                        cubic_bp(r[0].z, r[1].z, r[2].z, r[3].z, dy),
+                        cubic_bp(r[0].w, r[1].w, r[2].w, r[3].w, dy)};
   const float g = cubic_bp(arr[0], arr[1], arr[2], arr[3], dx);
   const size_t idx = vidx(x, y, z, W, H);
   if (mode == 0) {
@@ -617,7 +644,8 @@ __global__ __launch_bounds__(256) void k_bp_voxel(float* __restrict__ vol, int W
 hipError_t wapk_project(const float* vol, size_t vol_stride, const int* dims, float sd,
                         const WapView* views_dev, int nviews, int nu, int nv, float* proj,
                         float* raylen, float* weight, const float* b, float* py, float* corr,
-                        double alpha, int unroll, int mapping, hipStream_t s) {
+                        double alpha, int unroll, int mapping, const int* perm, int pbase,
+                        hipStream_t s) {
   // 32-bit byte offsets inside one volume
   if ((size_t)dims[0] * dims[1] * dims[2] >= (1ull << 30)) return hipErrorInvalidValue;
   if (mapping > 0) {
@@ -625,7 +653,7 @@ hipError_t wapk_project(const float* vol, size_t vol_stride, const int* dims, fl
 #define WAP_PR(R)                                                                              \
   hipLaunchKernelGGL(k_project_rays<R>, dim3(tiles, nviews), dim3(256), 0, s, vol, vol_stride, \
                      dims[0], dims[1], dims[2], sd, views_dev, nu, nv, proj, raylen, weight, b, \
-                     py, corr, alpha)
+                     py, corr, alpha, perm, pbase)
     switch (mapping) {
       case 2: WAP_PR(2); break;
       case 4: WAP_PR(4); break;
@@ -639,7 +667,7 @@ hipError_t wapk_project(const float* vol, size_t vol_stride, const int* dims, fl
 #define WAP_PJ(U)                                                                              \
   hipLaunchKernelGGL(k_project<U>, dim3(tiles, nviews), dim3(256), 0, s, vol, vol_stride,     \
                      dims[0], dims[1], dims[2], sd, views_dev, nu, nv, proj, raylen, weight, b, \
-                     py, corr, alpha)
+                     py, corr, alpha, perm, pbase)
   switch (unroll) {
     case 1: WAP_PJ(1); break;
     case 2: WAP_PJ(2); break;
