@@ -389,10 +389,10 @@ __global__ __launch_bounds__(256) void k_project(
 // forward projection with R rays per wave, each on G = 64/R lanes that take G
 // consecutive samples of their ray at a time (R = 1: one ray per wave). When rays run
 // along x, the volume's fastest axis, a gather of one wave then touches one or two cache
-// lines instead of dozens. Bit-identical to k_project: each lane replays the sequential
-// t += sd / pos += step additions from its group's chunk start, and each ray's samples
-// are added to its sum one by one in order (through lane shuffles), as the
-// one-sample-at-a-time march does.
+// lines instead of dozens. Bit-identical to k_project: each lane makes the sequential
+// t += sd / pos += step additions itself (g of them from a segment start, then G per
+// chunk), and each ray's samples are added to its sum one by one in order (through lane
+// shuffles), as the one-sample-at-a-time march does.
 // grid: x = tiles of R x 4 pixels (u x v; wave w takes row 4*tile_v + w), y = view
 // ---------------------------------------------------------------------------------
 template <int R>
@@ -449,7 +449,8 @@ __global__ __launch_bounds__(256) void k_project_rays(
     }
   }
   bool in_seg = false;
-  float t = 0.0f, px = 0.0f, py_ = 0.0f, pz = 0.0f, tb = 0.0f;
+  // this lane's sample of the current chunk (sample g of G in its ray's group)
+  float tg = 0.0f, xg = 0.0f, yg = 0.0f, zg = 0.0f, tb = 0.0f;
   const unsigned long long gmask = G == 64 ? ~0ull : (((1ull << G) - 1ull) << base);
   while (__any(!done)) {
     if (!done && !in_seg) {
@@ -458,24 +459,24 @@ __global__ __launch_bounds__(256) void k_project_rays(
         tb = mmin(mmin(tMax[0], tMax[1]), tMax[2]);
         if (tb > tf) tb = tf;
         if (tn - t1 > span) t1 = tn;
-        t = ceilf((t1 - tn) / sd) * sd + tn;
-        px = r.z[0] + r.u[0] * t;
-        py_ = r.z[1] + r.u[1] * t;
-        pz = r.z[2] + r.u[2] * t;
+        // the segment start, then g additions to this lane's first sample
+        tg = ceilf((t1 - tn) / sd) * sd + tn;
+        xg = r.z[0] + r.u[0] * tg;
+        yg = r.z[1] + r.u[1] * tg;
+        zg = r.z[2] + r.u[2] * tg;
+#pragma unroll
+        for (int q = 0; q < G - 1; ++q) {
+          if (q < g) {
+            tg = tg + sd;
+            xg = xg + step[0];
+            yg = yg + step[1];
+            zg = zg + step[2];
+          }
+        }
         in_seg = true;
       } else {
         done = true;
       }
-    }
-    // this lane's sample: g additions from the chunk start
-    float tg = t, xg = px, yg = py_, zg = pz;
-#pragma unroll
-    for (int q = 0; q < G - 1; ++q) {
-      const bool adv = q < g;
-      tg = adv ? tg + sd : tg;
-      xg = adv ? xg + step[0] : xg;
-      yg = adv ? yg + step[1] : yg;
-      zg = adv ? zg + step[2] : zg;
     }
     const bool ok = !done && in_seg && tg < tb && visits + g < cap;
     const float val = trilinear32(V, xg, yg, zg, W, H, D) * sd;
@@ -487,14 +488,18 @@ __global__ __launch_bounds__(256) void k_project_rays(
     }
     sw += (float)count;
     visits += count;
-    const float tl = __shfl(tg, base + G - 1, 64), xl = __shfl(xg, base + G - 1, 64);
-    const float yl = __shfl(yg, base + G - 1, 64), zl = __shfl(zg, base + G - 1, 64);
     if (!done && in_seg) {
       if (count == G) {
-        t = tl + sd;
-        px = xl + step[0];
-        py_ = yl + step[1];
-        pz = zl + step[2];
+        // the next chunk: each lane's sample moves on by G steps, through the same
+        // sequence of additions as the one-by-one march (which passes through the other
+        // lanes' samples)
+#pragma unroll
+        for (int q = 0; q < G; ++q) {
+          tg = tg + sd;
+          xg = xg + step[0];
+          yg = yg + step[1];
+          zg = zg + step[2];
+        }
       } else {
         if (tMax[0] < tMax[1]) {
           if (tMax[0] < tMax[2])
