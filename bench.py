@@ -409,7 +409,7 @@ def cpu_baseline(L, S, NDET, sart, flow_iters, budget_s):
 def main():
     a = parse()
     if a.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1",
+        print(json.dumps({"error": "the benchmark runs on one GPU; --gpus must be 1",
                           "n_gpus": a.gpus}))
         sys.exit(2)
     import numpy as np
