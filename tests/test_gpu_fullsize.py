@@ -205,3 +205,30 @@ def test_fullsize_sart_sweeps_and_warp_pass(wctx, oracle, W, foam):
     assert got.max() > 0.0
     assert got.min() >= 0.0 and got.max() <= cfg.gmax
     assert not got[:, mask == 0].any()
+
+
+@pytest.mark.parametrize("storage,bar", [(1, 3e-3), (2, 5e-4)])
+def test_fullsize_precision_form_flow(wctx, oracle, foam, storage, bar):
+    """The precision form at full size: the benchmark's multiscale flow (5 scales, 3 warps
+    x 50 iterations) of a foam pair with the duals, gradient and ut stored in bfloat16 (1)
+    or half (2), against the float form (which equals the float oracle bit for bit). The
+    bars are the small-size bars against the FP64 oracle."""
+    shape = (L, L, L)
+    I1 = oracle.blur(foam, 0.8)
+    I2 = oracle.advect(helpers.smooth_flow(shape, 23, 2.0), I1, 1.0)
+    args = (0.008, 0.05, 50, 3, 0.5, 0.8, 5, [8] * 8)
+    prev = wctx.get_tuning("flow_storage")
+    try:
+        outs = {}
+        for sto in (0, storage):
+            wctx.set_tuning("flow_storage", sto)
+            out = torch.zeros((3,) + shape, device=DEV)
+            wctx.flow_multiscale(T(I1), T(I2), None, *args, out)
+            outs[sto] = H(out).astype(np.float64)
+    finally:
+        wctx.set_tuning("flow_storage", prev)
+    ref = outs[0]
+    err = float(np.linalg.norm((outs[storage] - ref).ravel()) / np.linalg.norm(ref.ravel()))
+    print(f"storage {storage}: 256^3 flow rel L2 vs the float form {err:.3e}", flush=True)
+    assert np.abs(ref).max() > 0.1
+    assert err <= bar
