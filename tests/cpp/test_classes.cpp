@@ -4,9 +4,9 @@
 //                      five rotation matrices about +y to OUT for a bitwise comparison
 //   test_classes cpu   host-only parts: CoherentVolume bricking, Scene camera chain
 //   test_classes gpu   PSART/TypicalRayProducer/VolumeData on the device: class-built
-//                      views equal wap_view_geometry bit for bit, SART_Proximal_Operator
-//                      and WarpAndProjct equal the C ABI drivers bit for bit, Render()
-//                      runs both temporal levels
+//                      views equal wap_view_geometry bit for bit, SART_Proximal_Operator,
+//                      WarpAndProjct and RenderContent equal the C ABI bit for bit,
+//                      Render() runs both temporal levels
 // Prints one "ok <name>" line per passed check and exits non-zero on the first failure.
 #include <hip/hip_runtime_api.h>
 #include <math.h>
@@ -88,12 +88,21 @@ static int cpu_checks() {
   return fails;
 }
 
+// exposes the protected per-view projection of PSART (a subclass's view of the class)
+struct PSARTProbe : public PSART {
+  using PSART::PSART;
+  void Content(int f, bool warped, int p, float rtime, int level) {
+    RenderContent(f, warped, p, rtime, level);
+  }
+  void Size(float s[3]) { GetSize3D(s); }
+};
+
 static int gpu_checks() {
   const int L = 24, NU = 40, NV = 36, NV_VIEWS = 12, NF = 3;
   int imgwh[2] = {NU, NV};
   int vols[3] = {L, L, L};
   double oxyz[3] = {0, 0, 0};
-  PSART ps(0.01, 0.05, 0.0, "out", imgwh, NV_VIEWS, 1, 1, 0.2, 4.0 * L, nullptr, oxyz, 6.0 * L,
+  PSARTProbe ps(0.01, 0.05, 0.0, "out", imgwh, NV_VIEWS, 1, 1, 0.2, 4.0 * L, nullptr, oxyz, 6.0 * L,
            1.0, "ATV", "Voxelbased", vols, NF, nullptr, 1);
   std::vector<double> ang(NV_VIEWS);
   for (int i = 0; i < NV_VIEWS; ++i) ang[i] = fmod(i * 0.6180339887498949, 1.0) * 360.0;
@@ -192,6 +201,52 @@ static int gpu_checks() {
   HC(hipMemcpy(lv2.data(), cv, lv2.size() * sizeof(float), hipMemcpyDeviceToHost));
   CHECK(memcmp(lv.data(), lv2.data(), lv.size() * sizeof(float)) == 0,
         "psart_warp_and_project_equals_c_abi");
+  // RenderContent: the class's projection of a plain and of a warped state == the C ABI
+  {
+    std::vector<float> img(nd), ref(nd), rl(nd), rlref(nd), flow(3 * n);
+    for (size_t q = 0; q < flow.size(); ++q) flow[q] = q < n ? 0.6f : (q < 2 * n ? -0.3f : 0.2f);
+    HC(hipMemcpy(vd.m_Frameslist[0][0].fFlow.data(), flow.data(), flow.size() * sizeof(float),
+                 hipMemcpyHostToDevice));
+    float *dp, *dr, *dw;
+    HC(hipMalloc(&dp, nd * sizeof(float)));
+    HC(hipMalloc(&dr, nd * sizeof(float)));
+    HC(hipMalloc(&dw, n * sizeof(float)));
+    const float* v1 = vd.m_Frameslist[0][1].fVolume.data();
+    ps.Content(1, false, 3, 0.0f, 0);
+    HC(hipMemcpy(img.data(), ps.RayProducer()->GetImageData(), nd * sizeof(float),
+                 hipMemcpyDeviceToHost));
+    HC(hipMemcpy(rl.data(), ps.RayProducer()->GetImageRaylength(), nd * sizeof(float),
+                 hipMemcpyDeviceToHost));
+    wap_forward_project(ps.Context(), v1, 0, d3, sd, 3, 1, NU, NV, dp, dr, nullptr, nullptr);
+    HC(hipMemcpy(ref.data(), dp, nd * sizeof(float), hipMemcpyDeviceToHost));
+    HC(hipMemcpy(rlref.data(), dr, nd * sizeof(float), hipMemcpyDeviceToHost));
+    CHECK(memcmp(img.data(), ref.data(), nd * sizeof(float)) == 0 &&
+              memcmp(rl.data(), rlref.data(), nd * sizeof(float)) == 0,
+          "psart_render_content_equals_c_abi");
+    // a negative time takes the flow into the state (state 0's flow for state 1)
+    ps.Content(1, true, 5, -0.25f, 0);
+    HC(hipMemcpy(img.data(), ps.RayProducer()->GetImageData(), nd * sizeof(float),
+                 hipMemcpyDeviceToHost));
+    wap_advect(ps.Context(), d3, aabb, vd.m_Frameslist[0][0].fFlow.data(), v1, 1, -0.25f, dw,
+               nullptr);
+    wap_forward_project(ps.Context(), dw, 0, d3, sd, 5, 1, NU, NV, dp, nullptr, nullptr,
+                        nullptr);
+    HC(hipMemcpy(ref.data(), dp, nd * sizeof(float), hipMemcpyDeviceToHost));
+    bool moved = false;
+    std::vector<float> plain(nd);
+    wap_forward_project(ps.Context(), v1, 0, d3, sd, 5, 1, NU, NV, dp, nullptr, nullptr,
+                        nullptr);
+    HC(hipMemcpy(plain.data(), dp, nd * sizeof(float), hipMemcpyDeviceToHost));
+    for (size_t q = 0; q < nd; ++q) moved = moved || plain[q] != ref[q];
+    CHECK(memcmp(img.data(), ref.data(), nd * sizeof(float)) == 0 && moved,
+          "psart_render_content_warped_equals_c_abi");
+    float sz[3];
+    ps.Size(sz);
+    CHECK(sz[0] == (float)L && sz[1] == (float)L && sz[2] == (float)L, "psart_size3d");
+    HC(hipFree(dp));
+    HC(hipFree(dr));
+    HC(hipFree(dw));
+  }
   // the headless outer loop over both temporal levels
   ps.nb_volOpt[0] = 1;
   ps.nb_volOpt[1] = 1;

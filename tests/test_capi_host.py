@@ -172,9 +172,9 @@ def test_classes_host_parts():
 
 @pytest.mark.gpu
 def test_classes_on_device():
-    """PSART views, SART_Proximal_Operator and WarpAndProjct equal the C ABI bit for bit;
-    Render() runs both temporal levels."""
+    """PSART views, SART_Proximal_Operator, WarpAndProjct and RenderContent (plain and
+    warped) equal the C ABI bit for bit; Render() runs both temporal levels."""
     r = subprocess.run([_classes_bin(), "gpu"], capture_output=True, text=True, timeout=300)
     print(r.stdout, r.stderr)
     assert r.returncode == 0, r.stdout + r.stderr
-    assert r.stdout.count("ok ") >= 7
+    assert r.stdout.count("ok ") >= 10

@@ -813,6 +813,33 @@ void PSART::SART_Proximal_Operator(int iframes, int nb_sart, int outloop, int st
                             outloop, stIter, flevel));
 }
 
+void PSART::RenderContent(int frame, bool isWarped, int ProIdx, float rtime, int flevel) {
+  if (!m_Data || !m_RayProducer) return;
+  std::vector<Frame_item>& fr = m_Data->m_Frameslist[flevel];
+  const int nf = (int)fr.size();
+  const float* vol = fr[frame].fVolume.data();
+  DeviceImage warped;
+  if (isWarped) {
+    // the flow into the state for a negative time, out of it otherwise
+    const int fsel = rtime < 0.0f ? std::max(0, frame - 1) : std::min(nf - 1, frame);
+    const float aabb[6] = {0.0f, (float)m_voldims[0], 0.0f, (float)m_voldims[1], 0.0f,
+                           (float)m_voldims[2]};
+    warped = DeviceImage(m_voldims[0], m_voldims[1], m_voldims[2], 1, 0.0f);
+    if (!Ok(wap_advect(m_ctx, m_voldims, aabb, fr[fsel].fFlow.data(), vol, 1, rtime,
+                       warped.data(), nullptr)))
+      return;
+    vol = warped.data();
+  }
+  Ok(wap_forward_project(m_ctx, vol, 0, m_voldims, GetSampleDistance(), ProIdx, 1,
+                         m_RayProducer->m_ViewPort[0], m_RayProducer->m_ViewPort[1],
+                         m_RayProducer->GetImageData(), m_RayProducer->GetImageRaylength(),
+                         m_RayProducer->GetWeightOfImage(), nullptr));
+}
+
+void PSART::GetSize3D(float size[3]) {
+  for (int a = 0; a < 3; ++a) size[a] = (float)(m_voldims[a] * m_volspace);
+}
+
 void PSART::WarpAndProjct(int iproj, int iframes, int* dimension, float rtime, float* aabb,
                           int outloop, int warpiter, char* _file_header, int flevel) {
   (void)dimension;

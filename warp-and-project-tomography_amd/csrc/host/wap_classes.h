@@ -278,6 +278,16 @@ class PSART : public Scene {
   float flow_tol = 0.008f, flow_lambda = 0.05f, flow_eta = 0.5f, flow_sigma = 0.8f;
   int nlevels = 2;
   float G_Max = 10.0f;
+  TypicalRayProducer* RayProducer() { return m_RayProducer; }
+
+ protected:
+  // Forward projection of view ProIdx of state iframe at temporal level flevel into the
+  // ray producer's image, ray-length and weight buffers; with isWarped the state is first
+  // advected by rtime along its flow, as SingleScalarVolumePrepare chooses it
+  // (Project/WaP_Tomo.cpp:1686-1762, Project/SingleScalarVolume.core.h:366-442).
+  virtual void RenderContent(int iframe, bool isWarped, int ProIdx, float rtime, int flevel);
+  // volume extent in model units (Project/WaP_Tomo.cpp:1705-1718)
+  virtual void GetSize3D(float size[3]);
 
  private:
   void BuildViews();
