@@ -5,8 +5,9 @@
 //   test_classes cpu   host-only parts: CoherentVolume bricking, Scene camera chain
 //   test_classes gpu   PSART/TypicalRayProducer/VolumeData on the device: class-built
 //                      views equal wap_view_geometry bit for bit, SART_Proximal_Operator,
-//                      WarpAndProjct and RenderContent equal the C ABI bit for bit,
-//                      Render() runs both temporal levels
+//                      WarpAndProjct, RenderContent and the free kernels (advection::advect,
+//                      optical_flow::L1TVOpticalFlowNonlinear_MultiScale3D) equal the C ABI
+//                      bit for bit, Render() runs both temporal levels
 // Prints one "ok <name>" line per passed check and exits non-zero on the first failure.
 #include <hip/hip_runtime_api.h>
 #include <math.h>
@@ -243,6 +244,43 @@ static int gpu_checks() {
     float sz[3];
     ps.Size(sz);
     CHECK(sz[0] == (float)L && sz[1] == (float)L && sz[2] == (float)L, "psart_size3d");
+    // the free kernels on device images == the C ABI
+    const image& src = vd.m_Frameslist[0][1].fVolume;
+    const image& vel = vd.m_Frameslist[0][0].fFlow;
+    image adv = advection::advect<float, image>(d3, aabb, vel, src, -0.6f);
+    wap_advect(ps.Context(), d3, aabb, vel.data(), src.data(), 1, -0.6f, dw, nullptr);
+    std::vector<float> x1(n), x2(n);
+    adv.download(x1.data());
+    HC(hipMemcpy(x2.data(), dw, n * sizeof(float), hipMemcpyDeviceToHost));
+    CHECK(adv.spectrum() == 1 && memcmp(x1.data(), x2.data(), n * sizeof(float)) == 0,
+          "advection_advect_equals_c_abi");
+    image I1 = vd.m_Frameslist[0][1].fVolume, I2 = adv;  // a pair moved by a known flow
+    image V0(L, L, L, 3, 0.0f), Y1(L, L, L, 3, 0.0f), Y2(L, L, L, 3, 0.0f), Y3(L, L, L, 3, 0.0f);
+    image Y4(L, L, L, 1, 0.0f);
+    int cut[8] = {2, 2, 2, 2, 2, 2, 2, 2};
+    int dd[3] = {L, L, L};
+    image fl = optical_flow::L1TVOpticalFlowNonlinear_MultiScale3D<float, image>(
+        dd, I1, I2, V0, Y1, Y2, Y3, Y4, 0.008f, 0.05f, 10, 1, 2, 0.5f, 0.8f, 2, 1.0f, cut);
+    float* dfl;
+    HC(hipMalloc(&dfl, 3 * n * sizeof(float)));
+    wap_flow_multiscale(ps.Context(), d3, I1.data(), I2.data(), V0.data(), 0.008f, 0.05f, 10, 2,
+                        0.5f, 0.8f, 2, cut, dfl, nullptr);
+    std::vector<float> f1(3 * n), f2(3 * n);
+    fl.download(f1.data());
+    HC(hipMemcpy(f2.data(), dfl, f2.size() * sizeof(float), hipMemcpyDeviceToHost));
+    float fmax = 0.0f, dmax = 0.0f;
+    for (size_t q = 0; q < f2.size(); ++q) {
+      fmax = std::max(fmax, std::fabs(f2[q]));
+      dmax = std::max(dmax, std::fabs(f1[q] - f2[q]));
+    }
+    printf("flow: max |v| %g, largest difference %g\n", fmax, dmax);
+    CHECK(fl.spectrum() == 3 && memcmp(f1.data(), f2.data(), f1.size() * sizeof(float)) == 0 &&
+              fmax > 0.0f,
+          "optical_flow_multiscale_equals_c_abi");
+    image none = optical_flow::L1TVOpticalFlowNonlinear_MultiScale3D<float, image>(
+        dd, I1, I2, V0, Y1, Y2, Y3, Y4, 0.008f, 0.05f, 10, 4, 2, 0.5f, 0.8f, 2, 1.0f, cut);
+    CHECK(none.is_empty(), "optical_flow_huber_form_refused");
+    HC(hipFree(dfl));
     HC(hipFree(dp));
     HC(hipFree(dr));
     HC(hipFree(dw));

@@ -172,9 +172,11 @@ def test_classes_host_parts():
 
 @pytest.mark.gpu
 def test_classes_on_device():
-    """PSART views, SART_Proximal_Operator, WarpAndProjct and RenderContent (plain and
-    warped) equal the C ABI bit for bit; Render() runs both temporal levels."""
+    """PSART views, SART_Proximal_Operator, WarpAndProjct, RenderContent (plain and
+    warped) and the free kernels advection::advect and
+    optical_flow::L1TVOpticalFlowNonlinear_MultiScale3D equal the C ABI bit for bit;
+    Render() runs both temporal levels."""
     r = subprocess.run([_classes_bin(), "gpu"], capture_output=True, text=True, timeout=300)
     print(r.stdout, r.stderr)
     assert r.returncode == 0, r.stdout + r.stderr
-    assert r.stdout.count("ok ") >= 10
+    assert r.stdout.count("ok ") >= 13

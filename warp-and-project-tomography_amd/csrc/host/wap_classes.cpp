@@ -943,3 +943,68 @@ void PSART::Render() {
   }
   (void)hipDeviceSynchronize();
 }
+
+// ---------------------------------------------------------------------------------
+// free kernels (advection::advect, optical_flow::L1TVOpticalFlowNonlinear_MultiScale3D)
+// ---------------------------------------------------------------------------------
+namespace wap_host {
+
+// created on first use and held until the process ends (no teardown order to get wrong
+// against the HIP runtime's own)
+WapContext* shared_context() {
+  static WapContext* ctx = nullptr;
+  if (!ctx && wap_ctx_create(&ctx, PSART::DefaultDevice) != WAP_OK) ctx = nullptr;
+  return ctx;
+}
+
+static bool finished(WapContext* c, int r) {
+  if (r != WAP_OK) {
+    fprintf(stderr, "wap: %s\n", c ? wap_last_error(c) : "no device context");
+    return false;
+  }
+  return hipDeviceSynchronize() == hipSuccess;
+}
+
+static bool fits(const DeviceImage& im, const int* dim, int channels) {
+  return im.width() == dim[0] && im.height() == dim[1] && im.depth() == dim[2] &&
+         (channels <= 0 || im.spectrum() == channels) && !im.is_empty();
+}
+
+DeviceImage advect(const int* dim, const float* aabb, const DeviceImage& vel,
+                   const DeviceImage& src, float dt) {
+  WapContext* c = shared_context();
+  if (!c || !dim || !aabb || !fits(vel, dim, 3) || !fits(src, dim, 0)) {
+    fprintf(stderr, "wap: advect: the velocity must be (W,H,D,3) and the source (W,H,D,nc)\n");
+    return DeviceImage();
+  }
+  DeviceImage out(dim[0], dim[1], dim[2], src.spectrum(), 0.0f);
+  if (!finished(c, wap_advect(c, dim, aabb, vel.data(), src.data(), src.spectrum(), dt,
+                              out.data(), nullptr)))
+    return DeviceImage();
+  return out;
+}
+
+DeviceImage flow_multiscale(const int* dim, const DeviceImage& I1, const DeviceImage& I2,
+                            const DeviceImage& V_init, float tol, float lambda, int max_iter,
+                            int norm, int nwarps, float eta, float sigma, int scales,
+                            const int* cut) {
+  WapContext* c = shared_context();
+  if (norm == 4) {
+    fprintf(stderr, "wap: the Huber form of the flow (norm 4) is not supported\n");
+    return DeviceImage();
+  }
+  if (!c || !dim || !cut || !fits(I1, dim, 1) || !fits(I2, dim, 1) ||
+      (!V_init.is_empty() && !fits(V_init, dim, 3))) {
+    fprintf(stderr, "wap: flow: the images must be (W,H,D,1) and the initial flow (W,H,D,3)\n");
+    return DeviceImage();
+  }
+  DeviceImage out(dim[0], dim[1], dim[2], 3, 0.0f);
+  if (!finished(c, wap_flow_multiscale(c, dim, I1.data(), I2.data(),
+                                       V_init.is_empty() ? nullptr : V_init.data(), tol, lambda,
+                                       max_iter, nwarps, eta, sigma, scales, cut, out.data(),
+                                       nullptr)))
+    return DeviceImage();
+  return out;
+}
+
+}  // namespace wap_host

@@ -18,6 +18,7 @@
 #include <stddef.h>
 
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../../include/wap.h"
@@ -315,3 +316,57 @@ class PSART : public Scene {
 };
 
 typedef PSART WaP_Tomo;
+
+// ---------------------------------------------------------------------------------
+// The reference's free kernels, for float fields on the device. They run on a process-wide
+// context on PSART::DefaultDevice and block until the result is ready. On an error they
+// return an empty image and print the context's message.
+// ---------------------------------------------------------------------------------
+namespace wap_host {
+WapContext* shared_context();
+DeviceImage advect(const int* dim, const float* aabb, const DeviceImage& vel,
+                   const DeviceImage& src, float dt);
+DeviceImage flow_multiscale(const int* dim, const DeviceImage& I1, const DeviceImage& I2,
+                            const DeviceImage& V_init, float tol, float lambda, int max_iter,
+                            int norm, int nwarps, float eta, float sigma, int scales,
+                            const int* cut);
+}  // namespace wap_host
+
+namespace advection {
+// Semi-Lagrangian advection of src (W,H,D,nc) by vel (W,H,D,3) over dt inside the box
+// aabb (Project/advection.h:117-146).
+template <typename real, typename img>
+img advect(const int* dim, const real* aabb, const img& vel, const img& src, real dt) {
+  static_assert(std::is_same<real, float>::value && std::is_same<img, DeviceImage>::value,
+                "device advection takes float fields");
+  return wap_host::advect(dim, aabb, vel, src, dt);
+}
+}  // namespace advection
+
+namespace optical_flow {
+// The multiscale L1 flow from image1 to image2 (Project/optical_flow.h:1002-1126): a
+// blurred pyramid, the flow of the coarsest level from _inputV resized to it, each
+// finer level from the one below. Returns the full-size flow (W,H,D,3).
+// The duals start at zero: the reference's call site passes zero images
+// (Project/WaP_Tomo.cpp:707-733), and they are not read here. _norm 4 (Huber) is not
+// supported (an empty image is returned); every other value is the L1 form, as in the
+// reference, where _huber only enters the Huber form.
+template <typename real, typename img>
+img L1TVOpticalFlowNonlinear_MultiScale3D(int* dim, const img& image1, const img& image2,
+                                          img& _inputV, img& _inputY1, img& _inputY2,
+                                          img& _inputY3, img& _inputY4, real _tol,
+                                          real _lambda, int _maxIterations, int _norm,
+                                          int _numberOfWarps, real eta, real sigma, int scales,
+                                          real _huber, int* cut) {
+  static_assert(std::is_same<real, float>::value && std::is_same<img, DeviceImage>::value,
+                "device flow takes float fields");
+  (void)_inputY1;
+  (void)_inputY2;
+  (void)_inputY3;
+  (void)_inputY4;
+  (void)_huber;
+  return wap_host::flow_multiscale(dim, image1, image2, _inputV, _tol, _lambda,
+                                   _maxIterations, _norm, _numberOfWarps, eta, sigma, scales,
+                                   cut);
+}
+}  // namespace optical_flow
