@@ -131,6 +131,17 @@ static int gpu_checks() {
     same = same && memcmp(&v, ps.Views() + i, sizeof(WapView)) == 0;
   }
   CHECK(same, "psart_views_equal_c_abi_geometry");
+  ps.InitViews();  // rebuilding the table from the same angles changes nothing
+  same = true;
+  for (int i = 0; i < NV_VIEWS; ++i) {
+    WapView v;
+    wap_view_geometry(&g, ang[i], &v);
+    same = same && memcmp(&v, ps.Views() + i, sizeof(WapView)) == 0;
+  }
+  float bg[3];
+  ps.SetBackGroundColor(0.25f, 0.5f, 0.75f);
+  ps.GetBackGroundColor(bg[0], bg[1], bg[2]);
+  CHECK(same && bg[0] == 0.25f && bg[1] == 0.5f && bg[2] == 0.75f, "psart_init_views_and_colour");
   // synthetic projections of a blob through the C ABI projector
   const size_t n = (size_t)L * L * L, nd = (size_t)NU * NV;
   std::vector<float> blob(n);

@@ -179,4 +179,4 @@ def test_classes_on_device():
     r = subprocess.run([_classes_bin(), "gpu"], capture_output=True, text=True, timeout=300)
     print(r.stdout, r.stderr)
     assert r.returncode == 0, r.stdout + r.stderr
-    assert r.stdout.count("ok ") >= 13
+    assert r.stdout.count("ok ") >= 14

@@ -769,6 +769,22 @@ void PSART::SetSampleDistance(float sd) { m_RayCaster->SetSampleDistance(sd); }
 
 float PSART::GetSampleDistance() { return m_RayCaster->GetSampleDistance(); }
 
+void PSART::InitViews() {
+  if (m_Data && !m_angles.empty()) BuildViews();
+}
+
+void PSART::SetBackGroundColor(float r, float g, float b) {
+  m_BackgroundColor[0] = r;
+  m_BackgroundColor[1] = g;
+  m_BackgroundColor[2] = b;
+}
+
+void PSART::GetBackGroundColor(float& r, float& g, float& b) {
+  r = m_BackgroundColor[0];
+  g = m_BackgroundColor[1];
+  b = m_BackgroundColor[2];
+}
+
 void PSART::Init() {}
 
 void PSART::Resize(int width, int height) {

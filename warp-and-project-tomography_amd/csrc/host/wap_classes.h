@@ -246,8 +246,14 @@ class PSART : public Scene {
         double sdd, double ds, const char* prior, const char* bp, int* volxyz, int nframes,
         double* startdg, int nrounds);
   virtual ~PSART();
+  // the view table of every projection from the angles, once the data is set
+  // (Project/WaP_Tomo.cpp:1801-1850)
+  void InitViews();
   bool RotateVolume(int i);
   void SetData(VolumeData* data);
+  // kept for the GL viewer's clear colour (Project/WaP_Tomo.cpp:436-445); no effect here
+  void SetBackGroundColor(float r, float g, float b);
+  void GetBackGroundColor(float& r, float& g, float& b);
   void SART_Proximal_Operator(int iframes, int nb_sart, int outloop, int stIter, int flevel);
   void WarpAndProjct(int iproj, int iframes, int* dimension, float rtime, float* aabb,
                      int outloop, int warpiter, char* _file_header, int flevel);
@@ -308,6 +314,7 @@ class PSART : public Scene {
   float TV_w = 0.01f, TP_w = 0.05f, CP_tau = 1.0f, CP_sigma = 0.1125f;
   std::vector<double> m_angles;
   std::vector<WapView> m_views;
+  float m_BackgroundColor[3] = {0.0f, 0.0f, 0.0f};
   DeviceImage m_sino;   // projNo * H * W
   DeviceImage m_roiMask;
   int m_err = 0;
