@@ -117,7 +117,7 @@ struct WapContext {
   int perm_cap = 0;
   // concurrent flow solves of independent state pairs: worker streams (default
   // priority) and their events; `lane` selects the scratch set of the running worker
-  int flow_streams = 4;
+  int flow_streams = 6;
   std::vector<hipStream_t> workers;
   std::vector<hipEvent_t> worker_events;
   int lane = 0;
