@@ -57,10 +57,11 @@ def test_forward_project(wctx, oracle, W, parallel, shape, nu, nv):
     assert (Wt > 0).any()
 
 
-def test_project_correct_and_voxel_bp(wctx, oracle, W):
+@pytest.mark.parametrize("ds", [1.0, 0.8])
+def test_project_correct_and_voxel_bp(wctx, oracle, W, ds):
     shape, nu, nv = (24, 26, 28), 44, 40
     vol = helpers.blob_volume(shape, 2)
-    geo = helpers.cone_geometry(W, shape, nu, nv)
+    geo = helpers.cone_geometry(W, shape, nu, nv, ds=ds)
     views = W.views(geo, [30.0, 123.0])
     wctx.upload_views(views)
     b = np.random.default_rng(3).random((nv, nu)).astype(np.float32) * 5

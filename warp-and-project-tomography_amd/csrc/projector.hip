@@ -613,8 +613,12 @@ __global__ __launch_bounds__(256) void k_bp_voxel(float* __restrict__ vol, int W
     fx = (float)(v.half_w + (double)det[0]);
     fy = (float)(v.half_h + (double)det[1]);
   }
-  fx = (float)((double)fx / v.ds);
-  fy = (float)((double)fy / v.ds);
+  // a unit pitch leaves the double quotient equal to fx itself (the view-uniform branch
+  // skips two double divisions per voxel)
+  if (v.ds != 1.0) {
+    fx = (float)((double)fx / v.ds);
+    fy = (float)((double)fy / v.ds);
+  }
   const int px = (int)fx, py = (int)fy;
   const float dx = fx - (float)px, dy = fy - (float)py;
   if (px < 2 || px >= nu - 2 || py < 2 || py >= nv - 2) return;
