@@ -1,4 +1,5 @@
-"""A whole WaP outer iteration captured into a HIP graph and replayed.
+"""A whole WaP outer iteration captured into a HIP graph and replayed; the iteration on
+any number of worker streams; independent reconstructions run concurrently.
 
 After one warm-up call has settled the autotuners' choices and the scratch sizes,
 wap_iteration queues only kernels and event records/waits: the worker streams fork from
@@ -130,3 +131,34 @@ def test_uneven_reconstructions_run_concurrently(oracle, W):
         np.testing.assert_array_equal(vt, va)
         np.testing.assert_array_equal(ft_, fa)
         assert va.max() > 0.05
+
+
+@pytest.mark.parametrize("streams", [1, 2, 5, 8])
+def test_iteration_worker_streams_identical(wctx, oracle, W, streams):
+    """Seven states on 1 to 8 worker streams (each stream with its own scratch; the SART,
+    pyramid, pair-flow and warp phases fan out over them) give the oracle's floats."""
+    import pyoracle
+    shape, nstates, per, nu = (18, 20, 16), 7, 2, 28
+    cfg, views, sino, ws, we, ft, mask = _small_loop(W, oracle, shape, nstates, per, nu, 41)
+    prev = wctx.get_tuning("flow_streams")
+    wctx.set_tuning("flow_streams", streams)
+    try:
+        wctx.upload_views(views)
+        vols = torch.zeros((nstates,) + shape, device=DEV)
+        flows = torch.zeros((nstates, 3) + shape, device=DEV)
+        print(f"iteration on {streams} worker streams", flush=True)
+        for _ in range(2):
+            wctx.iteration(cfg, vols, flows, T(sino), ws, we, ft, T(mask))
+        ocfg = pyoracle.OrcLoop.from_buffer_copy(bytes(cfg))
+        oviews = (pyoracle.WapView * len(views)).from_buffer_copy(bytes(views))
+        rv = np.zeros((nstates,) + shape, np.float32)
+        rf = np.zeros((nstates, 3) + shape, np.float32)
+        for _ in range(2):
+            rv, rf = oracle.wap_iteration(ocfg, oviews, sino, np.asarray(ws, np.int32),
+                                          np.asarray(we, np.int32), np.asarray(ft, np.float32),
+                                          rv, rf, mask)
+        np.testing.assert_array_equal(H(flows), rf)
+        np.testing.assert_array_equal(H(vols), rv)
+        assert rv.max() > 0.05
+    finally:
+        wctx.set_tuning("flow_streams", prev)
