@@ -1211,7 +1211,7 @@ int wap_iteration(WapContext* c, const WapLoopConfig* L, float* vols, float* flo
   WAP_TRY(fan_out([&](int f, hipStream_t s) {
     return sart_prox_impl(c, L, vols + f * n, sino, ws[f], we[f], mask, s);
   }));
-  if (L->do_flow) {
+  if (L->do_flow && L->nstates > 1) {
     // every state's blurred pyramid once (each serves the pairs on both sides of it)
     PyrLevels P;
     WAP_TRY(pyr_levels(c, L->dims, L->flow_eta, L->flow_scales, &P));
