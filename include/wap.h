@@ -55,7 +55,9 @@ int wap_version(void);
  *   gradient u and ut (v, residual sums and all arithmetic stay float/double); forms
  *   1..5 of "flow_variant" need float storage.
  *   "flow_streams": worker streams for the independent pair flows of wap_iteration
- *   (1..8, default 6; each worker keeps its own scratch).
+ *   (1..8, default 6; each worker keeps its own scratch, about 50 floats per voxel for
+ *   the pair flows, so fewer workers run when that many sets do not fit in free device
+ *   memory; the results do not depend on the number).
  *   "project_mapping": how the forward projector maps lanes to rays: -1 (default) = per
  *   view from its ray direction, 0 = 8x8 pixel patches per wave, 2/4/8/16 = that many
  *   rays per wave each walking 64/n consecutive samples on 64/n lanes.

@@ -793,6 +793,20 @@ int pretune_levels(WapContext* c, const WapLoopConfig* L, const PyrLevels& P, co
   return rc;
 }
 
+// How many of K worker lanes fit in device memory, each with a scratch set of `per_lane`
+// bytes (a pair's flow solve keeps about 50 floats per voxel of the finest level). The
+// lanes' scratch already held from earlier calls counts as available; at least one lane.
+// The results do not depend on the lane count.
+int lanes_that_fit(WapContext* c, size_t per_lane, int K) {
+  size_t free_b = 0, total_b = 0;
+  if (K <= 1 || hipMemGetInfo(&free_b, &total_b) != hipSuccess) return K;
+  size_t held = 0;
+  for (const auto& kv : c->bufs)
+    if (kv.first.find('@') != std::string::npos) held += kv.second.bytes;
+  const size_t usable = (free_b + held) / 10 * 9;
+  return (int)std::max<size_t>(1, std::min<size_t>((size_t)K, usable / std::max<size_t>(per_lane, 1)));
+}
+
 int check_views(WapContext* c, int view0, int nviews) {
   if (!c->views || view0 < 0 || nviews <= 0 || view0 + nviews > c->nviews)
     return fail(c, WAP_E_STATE, "view range outside the uploaded view table");
@@ -1187,7 +1201,9 @@ int wap_iteration(WapContext* c, const WapLoopConfig* L, float* vols, float* flo
   // up to flow_streams worker streams, each with its own scratch, ordered after the work
   // queued before on the caller's stream and before the work queued after.
   auto fan_out = [&](auto&& per_state) -> int {
-    const int K = std::max(1, std::min(c->flow_streams, L->nstates));
+    // a state's SART sweeps, pyramid or warp pass keep about 6 floats per voxel
+    const int K = lanes_that_fit(c, (size_t)24 * n,
+                                 std::max(1, std::min(c->flow_streams, L->nstates)));
     if (K == 1) {
       for (int f = 0; f < L->nstates; ++f) WAP_TRY(per_state(f, st));
       return WAP_OK;
@@ -1223,7 +1239,8 @@ int wap_iteration(WapContext* c, const WapLoopConfig* L, float* vols, float* flo
     // the pairs' flows are independent: they run on up to flow_streams worker streams,
     // each with its own scratch, so one pair's small pyramid levels overlap another's
     // full-size level
-    const int K = std::max(1, std::min(c->flow_streams, L->nstates - 1));
+    const int K = lanes_that_fit(c, (size_t)200 * n,
+                                 std::max(1, std::min(c->flow_streams, L->nstates - 1)));
     WAP_TRY(pretune_levels(c, L, P, pyr, K > 1 ? 1 : 0, st));
     if (K == 1) {
       float* out;
