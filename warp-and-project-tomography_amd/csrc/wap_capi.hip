@@ -581,29 +581,34 @@ int flow_level(WapContext* c, const int* dims, const float* I1, const float* I2,
   WAP_TRY(scratch(c, "flow_ctl", 4, &ctl));
   WAP_TRY(scratch(c, "flow_err", 4, &err));
   // state: v in V / vB, y in yA / yB (y evolves everywhere and is written back to Y over
-  // the ROI at the end); u and ut stay zero outside the ROI
-  WAP_HIP(c, wapk_copy(vB, V, 3 * n, st));
-  if (sto == 0)
-    WAP_HIP(c, wapk_copy(yA, Y, 10 * n, st));
-  else
+  // the ROI at the end); u and ut stay zero outside the ROI. Every launch writes all of
+  // y, so float duals are read from Y by the first launch, which writes yB; from then on
+  // y alternates between yB and yA and Y is not written until the ROI write-back. The
+  // fused forms also write all of v; the two-kernel form (variant 0) writes v over the
+  // ROI only, so vB starts as a copy of V for it.
+  float* y0 = Y;
+  if (sto != 0) {
     WAP_HIP(c, wapk_to_store(sto, Y, yA, 10 * n, st));
+    y0 = yA;
+  }
   WAP_HIP(c, wapk_fill(u, fl4(3 * n), 0.0f, st));
   WAP_HIP(c, wapk_fill(ut, fl4(n), 0.0f, st));
   WAP_HIP(c, wapk_fill(kty, 3 * n, 0.0f, st));
   WAP_HIP(c, wapk_fill(kx4, n, 0.0f, st));
   int variant = c->flow_variant;
   if (variant < 0)
-    WAP_TRY(flow_tune(c, dims, roi, V, vB, yA, yB, u, ut, kx4, kty, ctl, st, &variant));
+    WAP_TRY(flow_tune(c, dims, roi, V, vB, y0, yB, u, ut, kx4, kty, ctl, st, &variant));
   c->flow_last = variant;
   int pair = c->flow_pair > 0 ? c->flow_pair - 1 : -1;
   if (c->flow_pair < 0)
-    WAP_TRY(pair_tune(c, dims, roi, V, vB, yA, yB, u, ut, kx4, kty, ctl, st, variant, &pair));
+    WAP_TRY(pair_tune(c, dims, roi, V, vB, y0, yB, u, ut, kx4, kty, ctl, st, variant, &pair));
   c->flow_pair_last = pair;
+  if (variant == 0) WAP_HIP(c, wapk_copy(vB, V, 3 * n, st));
   const size_t nblk = wapk_flow_blocks(variant, dims);
   WAP_TRY(scratch(c, "flow_pp", nblk, &pp));
   WAP_TRY(scratch(c, "flow_dp", nblk, &dp));
   const float sigma1 = std::min(1.0f / 3.0f, std::min(1.0f / 3.0f, 1.0f / 3.0f));
-  float *vcur = V, *vnext = vB, *ycur = yA, *ynext = yB;
+  float *vcur = V, *vnext = vB, *ycur = y0, *ynext = yB;
   int total = 0;
   std::vector<int> ctl_host(4);
   for (int w = 0; w < nwarps; ++w) {
@@ -638,6 +643,7 @@ int flow_level(WapContext* c, const int* dims, const float* I1, const float* I2,
       ++flips;
       std::swap(vcur, vnext);
       std::swap(ycur, ynext);
+      if (ynext == Y) ynext = yA;
     }
     WAP_HIP(c, wapk_flow_settle(sto, n, flips, vcur, vnext, ycur, ynext, ctl, st));
     if (iters_run) {
@@ -648,10 +654,11 @@ int flow_level(WapContext* c, const int* dims, const float* I1, const float* I2,
   }
   if (vcur != V) WAP_HIP(c, wapk_copy(V, vcur, 3 * n, st));
   if (has_roi) {
-    if (sto == 0)
-      WAP_HIP(c, wapk_copy_box(Y, ycur, dims, 10, lo, hi, st));
-    else
+    if (sto == 0) {
+      if (ycur != Y) WAP_HIP(c, wapk_copy_box(Y, ycur, dims, 10, lo, hi, st));
+    } else {
       WAP_HIP(c, wapk_box_from_store(sto, Y, ycur, dims, 10, lo, hi, st));
+    }
   }
   if (iters_run) *iters_run = total;
   return WAP_OK;
@@ -691,15 +698,12 @@ int pyr_levels(WapContext* c, const int* dims, float eta, int scales, PyrLevels*
 // full size, then every level resized from it (the full-size level is the blur itself).
 int build_pyramid(WapContext* c, const int* dims, const float* I, float sigma,
                   const PyrLevels& P, float* pyr, hipStream_t st) {
-  float* b;
-  WAP_TRY(scratch(c, "ms_blur", nvox(dims), &b));
+  // the finest level has the full extents (eta^0 = 1): blur straight into it
+  const PyrLevels::Lv& F = P.lv.back();
+  float* b = pyr + F.off;
   WAP_TRY(blur_impl(c, I, dims, 1, sigma, b, st));
-  for (const PyrLevels::Lv& L : P.lv) {
-    if (L.d[0] == dims[0] && L.d[1] == dims[1] && L.d[2] == dims[2])
-      WAP_HIP(c, wapk_copy(pyr + L.off, b, L.n, st));
-    else
-      WAP_TRY(resize_impl(c, b, dims, 1, L.d, pyr + L.off, st));
-  }
+  for (const PyrLevels::Lv& L : P.lv)
+    if (&L != &F) WAP_TRY(resize_impl(c, b, dims, 1, L.d, pyr + L.off, st));
   return WAP_OK;
 }
 
