@@ -143,13 +143,21 @@ def test_advect_multichannel_and_aabb(wctx, oracle):
 
 
 def test_blur_and_resize(wctx, oracle):
+    # line lengths below, at and above whole pairs of the blur's 8-sample chunks
+    for shp in [(33, 7, 48), (16, 1, 9)]:
+        im = helpers.blob_volume(shp, 5)
+        o = torch.zeros(shp, device=DEV)
+        wctx.blur(T(im), 1.3, o)
+        np.testing.assert_array_equal(H(o), oracle.blur(im, 1.3))
     shape = (20, 18, 22)
     img = helpers.blob_volume(shape, 8)
     out = torch.zeros(shape, device=DEV)
     wctx.blur(T(img), 0.8, out)
     np.testing.assert_array_equal(H(out), oracle.blur(img, 0.8))
     img3 = np.stack([img, img * 2, -img])
-    for nd in [(10, 9, 11), (40, 36, 44), (5, 4, 5), (20, 7, 30)]:
+    # (6, 5, 300) and (3, 290, 7) (D, H, W): new extents, and x extents feeding the later
+    # passes, above one 256-thread row of the launch grid
+    for nd in [(10, 9, 11), (40, 36, 44), (5, 4, 5), (20, 7, 30), (6, 5, 300), (3, 290, 7)]:
         o = torch.zeros((3,) + nd, device=DEV)
         wctx.resize(T(img3), o)
         np.testing.assert_array_equal(H(o), oracle.resize(img3, nd))

@@ -16,7 +16,95 @@ struct Deriche {
   float a0, a1, a2, a3, b1, b2, coefp, coefn;
 };
 
-// one line per thread: forward recursion into dst, backward recursion adds onto it
+// One line per thread: forward recursion into dst, backward recursion adds onto it.
+// The samples are read in chunks of 8 with the next chunk's loads in flight while the
+// current chunk's recursion runs (clamped indices, so the loads are unconditional); the
+// float operations are the reference's, in its order. Along x (XAXIS, a thread's line
+// is contiguous) chunks are read and written as 16-byte accesses.
+constexpr int kDC = 8;
+
+// samples m0 .. m0+7 (index clamped to N-1)
+template <bool XAXIS>
+WAP_DEV void ld_up(float* v, const float* __restrict__ a, int m0, int N, size_t off) {
+  if (XAXIS && m0 + kDC <= N) {
+    __builtin_memcpy(v, a + m0, sizeof(float) * kDC);
+  } else {
+#pragma unroll
+    for (int k = 0; k < kDC; ++k) v[k] = a[(size_t)min(m0 + k, N - 1) * off];
+  }
+}
+
+// samples top, top-1, ..., top-7 (index clamped at 0), highest first
+template <bool XAXIS>
+WAP_DEV void ld_down(float* v, const float* __restrict__ a, int top, size_t off) {
+  if (XAXIS && top - kDC + 1 >= 0) {
+    float t[kDC];
+    __builtin_memcpy(t, a + (top - kDC + 1), sizeof(float) * kDC);
+#pragma unroll
+    for (int k = 0; k < kDC; ++k) v[k] = t[kDC - 1 - k];
+  } else {
+#pragma unroll
+    for (int k = 0; k < kDC; ++k) v[k] = a[(size_t)max(top - k, 0) * off];
+  }
+}
+
+// forward recursion over samples m0 .. m0+7 (those below N; all of them without GUARD)
+template <bool XAXIS, bool GUARD>
+WAP_DEV void deriche_fwd(const Deriche& c, const float* v, float* __restrict__ y, int m0, int N,
+                         size_t off, float& xp, float& yp, float& yb) {
+  float out[kDC];
+#pragma unroll
+  for (int k = 0; k < kDC; ++k) {
+    if (!GUARD || m0 + k < N) {
+      const float xc = v[k];
+      const float yc = c.a0 * xc + c.a1 * xp - c.b1 * yp - c.b2 * yb;
+      out[k] = yc;
+      xp = xc;
+      yb = yp;
+      yp = yc;
+    }
+  }
+  if (XAXIS && (!GUARD || m0 + kDC <= N)) {
+    __builtin_memcpy(y + m0, out, sizeof(float) * kDC);
+  } else {
+#pragma unroll
+    for (int k = 0; k < kDC; ++k)
+      if (!GUARD || m0 + k < N) y[(size_t)(m0 + k) * off] = out[k];
+  }
+}
+
+// backward recursion over samples top .. top-7 (those >= 0; all of them without GUARD),
+// adding onto the forward results yf of the same samples
+template <bool XAXIS, bool GUARD>
+WAP_DEV void deriche_bwd(const Deriche& c, const float* v, const float* yf,
+                         float* __restrict__ y, int top, size_t off, float& xn, float& xa,
+                         float& yn, float& ya) {
+  float out[kDC];
+#pragma unroll
+  for (int k = 0; k < kDC; ++k) {
+    if (!GUARD || top - k >= 0) {
+      const float xc = v[k];
+      const float yc = c.a2 * xn + c.a3 * xa - c.b1 * yn - c.b2 * ya;
+      xa = xn;
+      xn = xc;
+      ya = yn;
+      yn = yc;
+      out[k] = yf[k] + yc;
+    }
+  }
+  if (XAXIS && (!GUARD || top - kDC + 1 >= 0)) {
+    float t[kDC];
+#pragma unroll
+    for (int k = 0; k < kDC; ++k) t[kDC - 1 - k] = out[k];
+    __builtin_memcpy(y + (top - kDC + 1), t, sizeof(float) * kDC);
+  } else {
+#pragma unroll
+    for (int k = 0; k < kDC; ++k)
+      if (!GUARD || top - k >= 0) y[(size_t)(top - k) * off] = out[k];
+  }
+}
+
+template <bool XAXIS>
 __global__ __launch_bounds__(256) void k_deriche(const float* __restrict__ src,
                                                  float* __restrict__ dst, int W, int H, int D,
                                                  int axis, Deriche c) {
@@ -25,7 +113,7 @@ __global__ __launch_bounds__(256) void k_deriche(const float* __restrict__ src,
   const size_t lines = axis == 0 ? (size_t)H * D : (axis == 1 ? (size_t)W * D : (size_t)W * H);
   const size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (l >= lines) return;
-  if (axis == 0) {
+  if (XAXIS) {
     N = W;
     off = 1;
     base = l * W;
@@ -41,52 +129,106 @@ __global__ __launch_bounds__(256) void k_deriche(const float* __restrict__ src,
   }
   const float* x = src + base;
   float* y = dst + base;
+  // two register chunks in turn: the next chunk's loads are issued before the current
+  // chunk's recursion; the main loops take whole pairs of chunks and the tails the rest
+  float A[kDC], B[kDC];
   float xp = x[0];
   float yb = c.coefp * xp, yp = yb;
-  for (int m = 0; m < N; ++m) {
-    const float xc = x[(size_t)m * off];
-    const float yc = c.a0 * xc + c.a1 * xp - c.b1 * yp - c.b2 * yb;
-    y[(size_t)m * off] = yc;
-    xp = xc;
-    yb = yp;
-    yp = yc;
+  int m0 = 0;
+  ld_up<XAXIS>(A, x, 0, N, off);
+  for (; m0 + 2 * kDC <= N; m0 += 2 * kDC) {
+    ld_up<XAXIS>(B, x, m0 + kDC, N, off);
+    deriche_fwd<XAXIS, false>(c, A, y, m0, N, off, xp, yp, yb);
+    ld_up<XAXIS>(A, x, m0 + 2 * kDC, N, off);
+    deriche_fwd<XAXIS, false>(c, B, y, m0 + kDC, N, off, xp, yp, yb);
   }
+  if (m0 < N) {
+    deriche_fwd<XAXIS, true>(c, A, y, m0, N, off, xp, yp, yb);
+    if (m0 + kDC < N) {
+      ld_up<XAXIS>(B, x, m0 + kDC, N, off);
+      deriche_fwd<XAXIS, true>(c, B, y, m0 + kDC, N, off, xp, yp, yb);
+    }
+  }
+  float YA[kDC], YB[kDC];
   float xn = x[(size_t)(N - 1) * off], xa = xn;
   float yn = c.coefn * xn, ya = yn;
-  for (int m = N - 1; m >= 0; --m) {
-    const float xc = x[(size_t)m * off];
-    const float yc = c.a2 * xn + c.a3 * xa - c.b1 * yn - c.b2 * ya;
-    xa = xn;
-    xn = xc;
-    ya = yn;
-    yn = yc;
-    y[(size_t)m * off] = y[(size_t)m * off] + yc;
+  int top = N - 1;
+  ld_down<XAXIS>(A, x, top, off);
+  ld_down<XAXIS>(YA, y, top, off);
+  for (; top - 2 * kDC + 1 >= 0; top -= 2 * kDC) {
+    ld_down<XAXIS>(B, x, top - kDC, off);
+    ld_down<XAXIS>(YB, y, top - kDC, off);
+    deriche_bwd<XAXIS, false>(c, A, YA, y, top, off, xn, xa, yn, ya);
+    // samples at or below top - 16 (clamped at 0): none is written before it is read
+    ld_down<XAXIS>(A, x, top - 2 * kDC, off);
+    ld_down<XAXIS>(YA, y, top - 2 * kDC, off);
+    deriche_bwd<XAXIS, false>(c, B, YB, y, top - kDC, off, xn, xa, yn, ya);
+  }
+  if (top >= 0) {
+    deriche_bwd<XAXIS, true>(c, A, YA, y, top, off, xn, xa, yn, ya);
+    if (top - kDC >= 0) {
+      ld_down<XAXIS>(B, x, top - kDC, off);
+      ld_down<XAXIS>(YB, y, top - kDC, off);
+      deriche_bwd<XAXIS, true>(c, B, YB, y, top - kDC, off, xn, xa, yn, ya);
+    }
   }
 }
 
 // Moving-average shrink along one axis. For output t the host table lists the source
 // indices and integer weights in the reference's visiting order: tmp = sum(x*d)/n.
+// x points at the line's first sample; stride is the axis stride in floats.
+WAP_DEV float avg_shrink(const float* __restrict__ x, size_t stride, int n, int t,
+                         const int* __restrict__ tstart, const int* __restrict__ tidx,
+                         const float* __restrict__ tw) {
+  float acc = 0.0f;
+  const int q0 = tstart[t], q1 = tstart[t + 1];
+  int q = q0;
+  // four taps at a time, loads first, then the sums in the reference's order
+  for (; q + 4 <= q1; q += 4) {
+    float v[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) v[k] = x[(size_t)tidx[q + k] * stride];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) acc += v[k] * tw[q + k];
+  }
+  for (; q < q1; ++q) acc += x[(size_t)tidx[q] * stride] * tw[q];
+  return acc / (float)n;
+}
+
+// Catmull-Rom growth along one axis with the reference's cumulative offset table
+// (source index p in [0, n-1] and fraction t per output, accumulated in double on the
+// host). The reference's edge rules (v0 = v1 at p = 0, v2 = v1 at p = n-1, v3 = v2 at
+// p >= n-2) are the taps p-1..p+2 with the index clamped to [0, n-1], so the four loads
+// are unconditional and issue together.
+WAP_DEV float cubic_grow(const float* __restrict__ x, size_t stride, int n, int p, double t) {
+  const double v0 = (double)x[(size_t)max(p - 1, 0) * stride];
+  const double v1 = (double)x[(size_t)p * stride];
+  const double v2 = (double)x[(size_t)min(p + 1, n - 1) * stride];
+  const double v3 = (double)x[(size_t)min(p + 2, n - 1) * stride];
+  const double val = v1 + 0.5f * (t * (-v0 + v2) + t * t * (2 * v0 - 5 * v1 + 4 * v2 - v3) +
+                                  t * t * t * (-v0 + 3 * v1 - 3 * v2 + v3));
+  const double lo = -3.4028234663852886e38, hi = 3.4028234663852886e38;
+  return (float)(val < lo ? lo : (val > hi ? hi : val));
+}
+
+// Element e = (o, t, i) of the output: o over outer lines, t over the new axis, i over
+// the inner extent. The general kernels take one element per thread from a flat index
+// (64-bit divisions); the ordinary shapes use the grid forms below.
 __global__ __launch_bounds__(256) void k_resize_avg(const float* __restrict__ src,
                                                     float* __restrict__ dst, int n, int s,
                                                     size_t inner, size_t outer,
                                                     const int* __restrict__ tstart,
                                                     const int* __restrict__ tidx,
                                                     const float* __restrict__ tw) {
-  // element e = (o, t, i): o over outer lines, t over the new axis, i over inner
   const size_t total = outer * (size_t)s * inner;
   const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= total) return;
   const size_t i = e % inner;
   const int t = (int)((e / inner) % s);
   const size_t o = e / (inner * s);
-  const float* x = src + o * (size_t)n * inner + i;
-  float acc = 0.0f;
-  for (int q = tstart[t]; q < tstart[t + 1]; ++q) acc += x[(size_t)tidx[q] * inner] * tw[q];
-  dst[e] = acc / (float)n;
+  dst[e] = avg_shrink(src + o * (size_t)n * inner + i, inner, n, t, tstart, tidx, tw);
 }
 
-// Catmull-Rom growth along one axis with the reference's cumulative offset table
-// (source index p and fraction t per output, accumulated in double on the host).
 __global__ __launch_bounds__(256) void k_resize_cubic(const float* __restrict__ src,
                                                       float* __restrict__ dst, int n, int s,
                                                       size_t inner, size_t outer,
@@ -98,17 +240,52 @@ __global__ __launch_bounds__(256) void k_resize_cubic(const float* __restrict__ 
   const size_t i = e % inner;
   const int xo = (int)((e / inner) % s);
   const size_t o = e / (inner * s);
-  const float* x = src + o * (size_t)n * inner + i;
-  const int p = pidx[xo];
-  const double t = frac[xo];
-  const double v1 = (double)x[(size_t)p * inner];
-  const double v0 = p > 0 ? (double)x[(size_t)(p - 1) * inner] : v1;
-  const double v2 = p <= n - 2 ? (double)x[(size_t)(p + 1) * inner] : v1;
-  const double v3 = p < n - 2 ? (double)x[(size_t)(p + 2) * inner] : v2;
-  const double val = v1 + 0.5f * (t * (-v0 + v2) + t * t * (2 * v0 - 5 * v1 + 4 * v2 - v3) +
-                                  t * t * t * (-v0 + 3 * v1 - 3 * v2 + v3));
-  const double lo = -3.4028234663852886e38, hi = 3.4028234663852886e38;
-  dst[e] = (float)(val < lo ? lo : (val > hi ? hi : val));
+  dst[e] = cubic_grow(src + o * (size_t)n * inner + i, inner, n, pidx[xo], frac[xo]);
+}
+
+// Grid forms. Axis x (inner == 1): blockIdx.x is the line o and threads run along the
+// new axis. Other axes: blockIdx.x/threadIdx.x run along the inner extent, blockIdx.y is
+// the output index t (so the table entries are uniform and read once per wave) and
+// blockIdx.z is o; loads and stores are coalesced along the inner extent.
+template <bool ROWS>
+__global__ __launch_bounds__(256) void k_resize_avg_grid(const float* __restrict__ src,
+                                                         float* __restrict__ dst, int n, int s,
+                                                         unsigned inner,
+                                                         const int* __restrict__ tstart,
+                                                         const int* __restrict__ tidx,
+                                                         const float* __restrict__ tw) {
+  const unsigned a = (ROWS ? blockIdx.y : blockIdx.x) * 256u + threadIdx.x;
+  if (ROWS) {
+    if (a >= (unsigned)s) return;
+    const size_t o = blockIdx.x;
+    dst[o * s + a] = avg_shrink(src + o * n, 1, n, (int)a, tstart, tidx, tw);
+  } else {
+    if (a >= inner) return;
+    const int t = blockIdx.y;
+    const size_t o = blockIdx.z;
+    dst[(o * s + t) * inner + a] =
+        avg_shrink(src + o * (size_t)n * inner + a, inner, n, t, tstart, tidx, tw);
+  }
+}
+
+template <bool ROWS>
+__global__ __launch_bounds__(256) void k_resize_cubic_grid(const float* __restrict__ src,
+                                                           float* __restrict__ dst, int n,
+                                                           int s, unsigned inner,
+                                                           const int* __restrict__ pidx,
+                                                           const double* __restrict__ frac) {
+  const unsigned a = (ROWS ? blockIdx.y : blockIdx.x) * 256u + threadIdx.x;
+  if (ROWS) {
+    if (a >= (unsigned)s) return;
+    const size_t o = blockIdx.x;
+    dst[o * s + a] = cubic_grow(src + o * n, 1, n, pidx[a], frac[a]);
+  } else {
+    if (a >= inner) return;
+    const int t = blockIdx.y;
+    const size_t o = blockIdx.z;
+    dst[(o * s + t) * inner + a] =
+        cubic_grow(src + o * (size_t)n * inner + a, inner, n, pidx[t], frac[t]);
+  }
 }
 
 __global__ void k_fill(float* __restrict__ p, size_t n, float v) {
@@ -251,25 +428,58 @@ hipError_t wapk_deriche(const float* src, float* dst, const int* dims, int axis,
   Deriche c{coef8[0], coef8[1], coef8[2], coef8[3], coef8[4], coef8[5], coef8[6], coef8[7]};
   const size_t lines = axis == 0 ? (size_t)dims[1] * dims[2]
                                  : (axis == 1 ? (size_t)dims[0] * dims[2] : (size_t)dims[0] * dims[1]);
-  hipLaunchKernelGGL(k_deriche, dim3(nblk(lines)), dim3(256), 0, s, src, dst, dims[0], dims[1],
-                     dims[2], axis, c);
+  if (axis == 0)
+    hipLaunchKernelGGL(k_deriche<true>, dim3(nblk(lines)), dim3(256), 0, s, src, dst, dims[0],
+                       dims[1], dims[2], axis, c);
+  else
+    hipLaunchKernelGGL(k_deriche<false>, dim3(nblk(lines)), dim3(256), 0, s, src, dst, dims[0],
+                       dims[1], dims[2], axis, c);
   return hipGetLastError();
+}
+
+// which resize launch fits: 1 = lines along x, 2 = (inner, t, o) grid, 0 = flat index
+inline int resize_form(int sz, size_t inner, size_t outer) {
+  const size_t chunks = (size_t)(sz + 255) / 256;
+  if (inner == 1 && outer <= 0x7fffffffu && chunks <= 65535) return 1;
+  if (inner > 1 && inner <= 0x7fffff00u && sz <= 65535 && outer <= 65535) return 2;
+  return 0;
 }
 
 hipError_t wapk_resize_avg(const float* src, float* dst, int n, int sz, size_t inner,
                            size_t outer, const int* tstart, const int* tidx, const float* tw,
                            hipStream_t s) {
   const size_t total = outer * (size_t)sz * inner;
-  hipLaunchKernelGGL(k_resize_avg, dim3(nblk(total)), dim3(256), 0, s, src, dst, n, sz, inner,
-                     outer, tstart, tidx, tw);
+  if (total == 0) return hipSuccess;
+  const int form = resize_form(sz, inner, outer);
+  if (form == 1)
+    hipLaunchKernelGGL(k_resize_avg_grid<true>, dim3((unsigned)outer, (unsigned)((sz + 255) / 256)),
+                       dim3(256), 0, s, src, dst, n, sz, 1u, tstart, tidx, tw);
+  else if (form == 2)
+    hipLaunchKernelGGL(k_resize_avg_grid<false>,
+                       dim3((unsigned)((inner + 255) / 256), (unsigned)sz, (unsigned)outer),
+                       dim3(256), 0, s, src, dst, n, sz, (unsigned)inner, tstart, tidx, tw);
+  else
+    hipLaunchKernelGGL(k_resize_avg, dim3(nblk(total)), dim3(256), 0, s, src, dst, n, sz, inner,
+                       outer, tstart, tidx, tw);
   return hipGetLastError();
 }
 
 hipError_t wapk_resize_cubic(const float* src, float* dst, int n, int sz, size_t inner,
                              size_t outer, const int* pidx, const double* frac, hipStream_t s) {
   const size_t total = outer * (size_t)sz * inner;
-  hipLaunchKernelGGL(k_resize_cubic, dim3(nblk(total)), dim3(256), 0, s, src, dst, n, sz, inner,
-                     outer, pidx, frac);
+  if (total == 0) return hipSuccess;
+  const int form = resize_form(sz, inner, outer);
+  if (form == 1)
+    hipLaunchKernelGGL(k_resize_cubic_grid<true>,
+                       dim3((unsigned)outer, (unsigned)((sz + 255) / 256)), dim3(256), 0, s, src,
+                       dst, n, sz, 1u, pidx, frac);
+  else if (form == 2)
+    hipLaunchKernelGGL(k_resize_cubic_grid<false>,
+                       dim3((unsigned)((inner + 255) / 256), (unsigned)sz, (unsigned)outer),
+                       dim3(256), 0, s, src, dst, n, sz, (unsigned)inner, pidx, frac);
+  else
+    hipLaunchKernelGGL(k_resize_cubic, dim3(nblk(total)), dim3(256), 0, s, src, dst, n, sz,
+                       inner, outer, pidx, frac);
   return hipGetLastError();
 }
 
