@@ -200,15 +200,25 @@ WAP_DEV float avg_shrink(const float* __restrict__ x, size_t stride, int n, int 
 // host). The reference's edge rules (v0 = v1 at p = 0, v2 = v1 at p = n-1, v3 = v2 at
 // p >= n-2) are the taps p-1..p+2 with the index clamped to [0, n-1], so the four loads
 // are unconditional and issue together.
-WAP_DEV float cubic_grow(const float* __restrict__ x, size_t stride, int n, int p, double t) {
-  const double v0 = (double)x[(size_t)max(p - 1, 0) * stride];
-  const double v1 = (double)x[(size_t)p * stride];
-  const double v2 = (double)x[(size_t)min(p + 1, n - 1) * stride];
-  const double v3 = (double)x[(size_t)min(p + 2, n - 1) * stride];
+WAP_DEV void grow_taps(const float* __restrict__ x, size_t stride, int n, int p, float* v) {
+  v[0] = x[(size_t)max(p - 1, 0) * stride];
+  v[1] = x[(size_t)p * stride];
+  v[2] = x[(size_t)min(p + 1, n - 1) * stride];
+  v[3] = x[(size_t)min(p + 2, n - 1) * stride];
+}
+
+WAP_DEV float grow_eval(const float* v, double t) {
+  const double v0 = v[0], v1 = v[1], v2 = v[2], v3 = v[3];
   const double val = v1 + 0.5f * (t * (-v0 + v2) + t * t * (2 * v0 - 5 * v1 + 4 * v2 - v3) +
                                   t * t * t * (-v0 + 3 * v1 - 3 * v2 + v3));
   const double lo = -3.4028234663852886e38, hi = 3.4028234663852886e38;
   return (float)(val < lo ? lo : (val > hi ? hi : val));
+}
+
+WAP_DEV float cubic_grow(const float* __restrict__ x, size_t stride, int n, int p, double t) {
+  float v[4];
+  grow_taps(x, stride, n, p, v);
+  return grow_eval(v, t);
 }
 
 // Element e = (o, t, i) of the output: o over outer lines, t over the new axis, i over
@@ -268,23 +278,46 @@ __global__ __launch_bounds__(256) void k_resize_avg_grid(const float* __restrict
   }
 }
 
+// The cubic grid forms take kRT outputs per thread (along x: the same output index on
+// kRT consecutive lines; otherwise kRT consecutive t), with all taps loaded before any
+// output is formed. Line or output indices past the end are clamped to the last one,
+// which is then formed and stored again with the same value, so every load and store is
+// unconditional.
+constexpr int kRT = 4;
+
 template <bool ROWS>
 __global__ __launch_bounds__(256) void k_resize_cubic_grid(const float* __restrict__ src,
                                                            float* __restrict__ dst, int n,
                                                            int s, unsigned inner,
+                                                           unsigned outer,
                                                            const int* __restrict__ pidx,
                                                            const double* __restrict__ frac) {
-  const unsigned a = (ROWS ? blockIdx.y : blockIdx.x) * 256u + threadIdx.x;
+  float v[kRT][4];
   if (ROWS) {
+    const unsigned a = blockIdx.y * 256u + threadIdx.x;
     if (a >= (unsigned)s) return;
-    const size_t o = blockIdx.x;
-    dst[o * s + a] = cubic_grow(src + o * n, 1, n, pidx[a], frac[a]);
+    const int p = pidx[a];
+    const double t = frac[a];
+    size_t o[kRT];
+#pragma unroll
+    for (int j = 0; j < kRT; ++j) o[j] = min(blockIdx.x * kRT + j, outer - 1);
+#pragma unroll
+    for (int j = 0; j < kRT; ++j) grow_taps(src + o[j] * n, 1, n, p, v[j]);
+#pragma unroll
+    for (int j = 0; j < kRT; ++j) dst[o[j] * s + a] = grow_eval(v[j], t);
   } else {
+    const unsigned a = blockIdx.x * 256u + threadIdx.x;
     if (a >= inner) return;
-    const int t = blockIdx.y;
+    const int t0 = (int)blockIdx.y * kRT;
     const size_t o = blockIdx.z;
-    dst[(o * s + t) * inner + a] =
-        cubic_grow(src + o * (size_t)n * inner + a, inner, n, pidx[t], frac[t]);
+    const float* x = src + o * (size_t)n * inner + a;
+    int t[kRT];
+#pragma unroll
+    for (int j = 0; j < kRT; ++j) t[j] = min(t0 + j, s - 1);
+#pragma unroll
+    for (int j = 0; j < kRT; ++j) grow_taps(x, inner, n, pidx[t[j]], v[j]);
+#pragma unroll
+    for (int j = 0; j < kRT; ++j) dst[(o * s + t[j]) * inner + a] = grow_eval(v[j], frac[t[j]]);
   }
 }
 
@@ -471,12 +504,14 @@ hipError_t wapk_resize_cubic(const float* src, float* dst, int n, int sz, size_t
   const int form = resize_form(sz, inner, outer);
   if (form == 1)
     hipLaunchKernelGGL(k_resize_cubic_grid<true>,
-                       dim3((unsigned)outer, (unsigned)((sz + 255) / 256)), dim3(256), 0, s, src,
-                       dst, n, sz, 1u, pidx, frac);
+                       dim3((unsigned)((outer + kRT - 1) / kRT), (unsigned)((sz + 255) / 256)),
+                       dim3(256), 0, s, src, dst, n, sz, 1u, (unsigned)outer, pidx, frac);
   else if (form == 2)
     hipLaunchKernelGGL(k_resize_cubic_grid<false>,
-                       dim3((unsigned)((inner + 255) / 256), (unsigned)sz, (unsigned)outer),
-                       dim3(256), 0, s, src, dst, n, sz, (unsigned)inner, pidx, frac);
+                       dim3((unsigned)((inner + 255) / 256), (unsigned)((sz + kRT - 1) / kRT),
+                            (unsigned)outer),
+                       dim3(256), 0, s, src, dst, n, sz, (unsigned)inner, (unsigned)outer, pidx,
+                       frac);
   else
     hipLaunchKernelGGL(k_resize_cubic, dim3(nblk(total)), dim3(256), 0, s, src, dst, n, sz,
                        inner, outer, pidx, frac);
