@@ -103,10 +103,20 @@ WAP_DEV float sample_taps(const float* __restrict__ img, const Taps& T) {
 }
 
 // flag = 1 if any of the n values is not finite (the flag is cleared beforehand)
+// four values per thread (one 16-byte load where all four exist)
 __global__ __launch_bounds__(256) void k_nonfinite(const float* __restrict__ a, size_t n,
                                                    int* __restrict__ flag) {
-  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < n && !isfinite(a[t])) *flag = 1;
+  const size_t t = 4 * ((size_t)blockIdx.x * blockDim.x + threadIdx.x);
+  if (t >= n) return;
+  bool bad = false;
+  if (t + 4 <= n) {
+    float4 v;
+    __builtin_memcpy(&v, a + t, sizeof(v));
+    bad = !isfinite(v.x) || !isfinite(v.y) || !isfinite(v.z) || !isfinite(v.w);
+  } else {
+    for (size_t q = t; q < n; ++q) bad = bad || !isfinite(a[q]);
+  }
+  if (bad) *flag = 1;
 }
 
 __global__ void k_clear_flag(int* flag) { *flag = 0; }
@@ -327,7 +337,7 @@ hipError_t wapk_advect(const int* dims, const float* aabb, const float* vel, con
   for (int a = 0; a < 6; ++a) G.bb[a] = aabb[a];
   const size_t n = (size_t)G.W * G.H * G.D;
   hipLaunchKernelGGL(k_clear_flag, dim3(1), dim3(1), 0, s, flag);
-  hipLaunchKernelGGL(k_nonfinite, dim3((unsigned)((3 * n + 255) / 256)), dim3(256), 0, s, vel,
+  hipLaunchKernelGGL(k_nonfinite, dim3((unsigned)((3 * n + 1023) / 1024)), dim3(256), 0, s, vel,
                      3 * n, flag);
   const size_t span = n * (size_t)(nc > 3 ? nc : 3);
   if (G.W >= 4 && G.H >= 3 && G.D >= 3 && span < (1ull << 31))

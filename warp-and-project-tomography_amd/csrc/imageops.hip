@@ -321,9 +321,15 @@ __global__ __launch_bounds__(256) void k_resize_cubic_grid(const float* __restri
   }
 }
 
+// fill and copy take four floats per thread (one 16-byte access where all four exist)
 __global__ void k_fill(float* __restrict__ p, size_t n, float v) {
-  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < n) p[t] = v;
+  const size_t t = 4 * ((size_t)blockIdx.x * blockDim.x + threadIdx.x);
+  if (t + 4 <= n) {
+    const float4 q = make_float4(v, v, v, v);
+    __builtin_memcpy(p + t, &q, sizeof(q));
+  } else {
+    for (size_t k = t; k < n; ++k) p[k] = v;
+  }
 }
 
 __global__ void k_fill_int(int* __restrict__ p, size_t n, int v) {
@@ -337,8 +343,14 @@ __global__ void k_fill_double(double* __restrict__ p, size_t n) {
 }
 
 __global__ void k_copy(float* __restrict__ d, const float* __restrict__ s, size_t n) {
-  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < n) d[t] = s[t];
+  const size_t t = 4 * ((size_t)blockIdx.x * blockDim.x + threadIdx.x);
+  if (t + 4 <= n) {
+    float4 q;
+    __builtin_memcpy(&q, s + t, sizeof(q));
+    __builtin_memcpy(d + t, &q, sizeof(q));
+  } else {
+    for (size_t k = t; k < n; ++k) d[k] = s[k];
+  }
 }
 
 // copy of the box [lo, hi) of nplanes planes
@@ -520,7 +532,7 @@ hipError_t wapk_resize_cubic(const float* src, float* dst, int n, int sz, size_t
 
 hipError_t wapk_fill(float* p, size_t n, float v, hipStream_t s) {
   if (n == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_fill, dim3(nblk(n)), dim3(256), 0, s, p, n, v);
+  hipLaunchKernelGGL(k_fill, dim3(nblk((n + 3) / 4)), dim3(256), 0, s, p, n, v);
   return hipGetLastError();
 }
 
@@ -532,7 +544,7 @@ hipError_t wapk_fill_double(double* p, size_t n, hipStream_t s) {
 
 hipError_t wapk_copy(float* d, const float* src, size_t n, hipStream_t s) {
   if (n == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_copy, dim3(nblk(n)), dim3(256), 0, s, d, src, n);
+  hipLaunchKernelGGL(k_copy, dim3(nblk((n + 3) / 4)), dim3(256), 0, s, d, src, n);
   return hipGetLastError();
 }
 
