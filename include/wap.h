@@ -44,7 +44,7 @@ int wap_version(void);
  *   "flow_variant": form of the optical-flow primal-dual iteration. -1 (default) = the
  *   built-in autotuner times the candidate forms on the first flow level of each grid
  *   size and keeps the fastest; 0 = two kernels (primal over the ROI, then dual over the
- *   grid); 1..5 = one kernel with LDS plane tiles; 6..29 = one single-pass kernel whose
+ *   grid); 1..5 = one kernel with LDS plane tiles; 6..31 = one single-pass kernel whose
  *   threads march z over column tiles (tile and chunk shapes listed in flow.hip).
  *   "flow_pair": two-iterations-per-pass form of the normal-mode iterations: -1
  *   (default) = autotuned per grid size (or off when single iterations are faster),

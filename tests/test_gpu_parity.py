@@ -163,6 +163,20 @@ def test_blur_and_resize(wctx, oracle):
         np.testing.assert_array_equal(H(o), oracle.resize(img3, nd))
 
 
+def test_resize_long_lines(wctx, oracle):
+    """More than 65535 lines behind a y pass: the launch falls back to the flat-index
+    kernels (growth 3 -> 5 and shrink 5 -> 3 along y, the other axes copied)."""
+    rng = np.random.default_rng(11)
+    img = rng.standard_normal((66000, 3, 2)).astype(np.float32)
+    o = torch.zeros((66000, 5, 2), device=DEV)
+    wctx.resize(T(img), o)
+    np.testing.assert_array_equal(H(o), oracle.resize(img, (66000, 5, 2)))
+    img5 = H(o).copy()
+    o3 = torch.zeros((66000, 3, 2), device=DEV)
+    wctx.resize(T(img5), o3)
+    np.testing.assert_array_equal(H(o3), oracle.resize(img5, (66000, 3, 2)))
+
+
 @pytest.mark.parametrize("max_iter,nwarps,roi", [(1, 1, [2] * 6), (2, 3, [2] * 6),
                                                   (50, 3, [2] * 6), (60, 1, [0] * 6),
                                                   (120, 2, [1, 2, 0, 3, 2, 1])])
@@ -252,7 +266,7 @@ def test_wap_iteration(wctx, oracle, W, parallel):
     assert rv.max() > 0.1
 
 
-@pytest.mark.parametrize("variant", list(range(30)))
+@pytest.mark.parametrize("variant", list(range(32)))
 def test_flow_iteration_variants_identical(wctx, oracle, variant):
     """Every primal-dual iteration form (two kernels, fused tiles) gives the oracle's
     floats, on a wide grid (64-column tiles) and a narrow one (16x16 tiles)."""

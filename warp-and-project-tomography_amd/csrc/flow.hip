@@ -986,7 +986,8 @@ static const ZmShape kZm[] = {
     {256, 1, 16, true, false},  {256, 1, 32, true, false},  {256, 2, 16, true, false},
     {256, 4, 16, true, false},  {256, 1, 8, true, false},   {256, 1, 64, true, false},
     {256, 2, 16, true, true},   {256, 4, 16, true, true},   {256, 2, 8, true, true},
-    {256, 4, 8, true, true},    {256, 2, 32, true, true},   {256, 4, 32, true, true}};
+    {256, 4, 8, true, true},    {256, 2, 32, true, true},   {256, 4, 32, true, true},
+    {256, 4, 4, true, true},    {256, 2, 4, true, true}};
 static const int kZmCount = (int)(sizeof(kZm) / sizeof(kZm[0]));
 
 static ZmShape zm_shape(int variant, const int* dims) {

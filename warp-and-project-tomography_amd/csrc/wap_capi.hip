@@ -463,7 +463,7 @@ int flow_tune(WapContext* c, const int* dims, const int* roi, const float* vcur,
     *chosen = it->second;
     return WAP_OK;
   }
-  static const int cand[] = {0, 15, 18, 20, 22, 24, 25, 26, 27, 28};
+  static const int cand[] = {0, 15, 18, 20, 22, 24, 25, 26, 27, 28, 30, 31};
   hipEvent_t e0, e1;
   WAP_HIP(c, hipEventCreate(&e0));
   WAP_HIP(c, hipEventCreate(&e1));
