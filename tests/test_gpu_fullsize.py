@@ -7,7 +7,9 @@ detector at 6L). The oracle finishes these cases in seconds on the host's cores:
 - advection of a 256^3 state;
 - a flow solve of 7 iterations x 2 warps, which runs single iterations and pair passes of
   the shape autotuned for 256^3;
-- the SART sweeps and the warp-and-project pass of a 4-state outer iteration.
+- the SART sweeps and the warp-and-project pass of a 4-state outer iteration;
+- the pyramid's Deriche blur of a 256^3 state, its shrink to 128^3, and the growth of a
+  3-channel 128^3 flow to 256^3.
 All of them must agree bit for bit. The size-independent properties:
 - equal images and a zero start keep the flow and its TV duals at zero (up to subnormals
   where the gradient is subnormal), and the residual test stops the solve at iteration 50;
@@ -50,6 +52,21 @@ def foam():
 def scan(W):
     geo = W.make_geometry((L, L, L), NU, NU, ds=1.0, sid=SID, sdd=SDD)
     return W.views(geo, ANGLES)
+
+
+def test_fullsize_blur_and_resize(wctx, oracle, foam):
+    b = torch.zeros((L, L, L), device=DEV)
+    wctx.blur(T(foam), 0.8, b)
+    ref = oracle.blur(foam, 0.8)
+    np.testing.assert_array_equal(H(b), ref)
+    half = (L // 2,) * 3
+    s = torch.zeros(half, device=DEV)
+    wctx.resize(b, s)
+    np.testing.assert_array_equal(H(s), oracle.resize(ref, half))
+    v = helpers.smooth_flow(half, 5, 2.0)
+    g = torch.zeros((3, L, L, L), device=DEV)
+    wctx.resize(T(v), g)
+    np.testing.assert_array_equal(H(g), oracle.resize(v, (L, L, L)))
 
 
 def test_fullsize_forward_projection(wctx, oracle, foam, scan):
