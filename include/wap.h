@@ -164,11 +164,13 @@ int wap_backproject_ray(WapContext* ctx, const float* corr, const int* dims, flo
                         int view0, int nviews, int nu, int nv, double scale, float* out,
                         float* wsum, void* stream);
 
-/* advection::advect (Project/advection.h:117-146): vel has 3 planes, src/out nc planes. */
+/* advection::advect (Project/advection.h:117-146): vel has 3 planes, src/out nc planes;
+ * out must not overlap src or vel (WAP_E_ARG). */
 int wap_advect(WapContext* ctx, const int* dims, const float* aabb, const float* vel,
                const float* src, int nc, float dt, float* out, void* stream);
 
-/* CImg get_blur(sigma) (Deriche, Neumann) of nc planes (Project/CImg.h:33249-33285). */
+/* CImg get_blur(sigma) (Deriche, Neumann) of nc planes (Project/CImg.h:33249-33285).
+ * out must not overlap src (WAP_E_ARG). */
 int wap_blur(WapContext* ctx, const float* src, const int* dims, int nc, float sigma,
              float* out, void* stream);
 

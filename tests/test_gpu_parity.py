@@ -154,6 +154,11 @@ def test_blur_and_resize(wctx, oracle):
     out = torch.zeros(shape, device=DEV)
     wctx.blur(T(img), 0.8, out)
     np.testing.assert_array_equal(H(out), oracle.blur(img, 0.8))
+    with pytest.raises(RuntimeError, match="overlap"):
+        wctx.blur(out, 0.8, out)
+    vel = torch.zeros((3,) + shape, device=DEV)
+    with pytest.raises(RuntimeError, match="overlap"):
+        wctx.advect(vel, out, -1.0, out, [0, shape[2], 0, shape[1], 0, shape[0]])
     img3 = np.stack([img, img * 2, -img])
     # (6, 5, 300) and (3, 290, 7) (D, H, W): new extents, and x extents feeding the later
     # passes, above one 256-thread row of the launch grid

@@ -1093,6 +1093,10 @@ int wap_advect(WapContext* c, const int* dims, const float* aabb, const float* v
     return fail(c, WAP_E_ARG, "wap_advect: every extent must be at least 3");
   if (!(aabb[1] > aabb[0] && aabb[3] > aabb[2] && aabb[5] > aabb[4]))
     return fail(c, WAP_E_ARG, "wap_advect: empty bounding box");
+  // every output voxel gathers from anywhere in src and vel
+  const size_t n = nvox(dims), lo = n * (size_t)nc;
+  if ((src < out + lo && out < src + lo) || (vel < out + lo && out < vel + 3 * n))
+    return fail(c, WAP_E_ARG, "wap_advect: out overlaps src or vel");
   WAP_TRY(advect_impl(c, dims, aabb, vel, src, nc, dt, out, S(stream)));
   return WAP_OK;
 }
@@ -1101,6 +1105,11 @@ int wap_blur(WapContext* c, const float* src, const int* dims, int nc, float sig
              void* stream) {
   if (!c || !src || !out || !dims_ok(dims) || nc <= 0 || sigma < 0.0f)
     return fail(c, WAP_E_ARG, "wap_blur: invalid arguments");
+  // each axis pass reads its input twice (forward and backward recursion), so the
+  // output may not share memory with the input
+  const size_t len = nvox(dims) * (size_t)nc;
+  if (src < out + len && out < src + len)
+    return fail(c, WAP_E_ARG, "wap_blur: src and out overlap");
   return blur_impl(c, src, dims, nc, sigma, out, S(stream));
 }
 
