@@ -43,13 +43,15 @@ def parse():
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-sample-seconds", type=float, default=20.0)
     p.add_argument("--profile-pd", action="store_true",
-                   help="run only the profiling workload of the PD iteration: one 8-iteration "
-                        "flow solve at full size, then a 13-plane copy of known size that "
+                   help="run only the profiling workload of the PD iteration: one flow solve "
+                        "at full size (--pd-iters), then a 13-plane copy of known size that "
                         "calibrates the HBM byte counters (use under rocprofv3)")
     p.add_argument("--pd-form", type=int, default=-1,
                    help="with --profile-pd: the PD iteration form (-1 = autotuned)")
     p.add_argument("--pd-pair", type=int, default=-1,
                    help="with --profile-pd: the pair-pass shape (-1 = autotuned, 0 = off)")
+    p.add_argument("--pd-iters", type=int, default=8,
+                   help="with --profile-pd: iterations of the flow solve (default 8)")
     p.add_argument("--profile-proj", action="store_true",
                    help="run only the profiling workload of the forward projector: one foam "
                         "state at full size projected along 8 views, then the same calibration "
@@ -426,8 +428,9 @@ def main():
         I2 = torch.rand((L, L, L), device=dev)
         V = torch.zeros((3, L, L, L), device=dev)
         Y = torch.zeros((10, L, L, L), device=dev)
-        # 8 iterations: one single, then four pair passes (when pairs are on)
-        ctx.flow_solve(I1, I2, V, Y, 0.0, 0.05, 8, 1, [0, L, 0, L, 0, L], [8] * 6)
+        # with the default 8 iterations: one single, then four pair passes (when pairs
+        # are on)
+        ctx.flow_solve(I1, I2, V, Y, 0.0, 0.05, a.pd_iters, 1, [0, L, 0, L, 0, L], [8] * 6)
         torch.cuda.synchronize()
         src = torch.rand(13 * L ** 3, device=dev)
         dst = torch.empty_like(src)
