@@ -56,6 +56,11 @@ def parse():
                    help="run only the profiling workload of the forward projector: one foam "
                         "state at full size projected along 8 views, then the same calibration "
                         "copy as --profile-pd (use under rocprofv3)")
+    p.add_argument("--profile-pyramid", action="store_true",
+                   help="run only the profiling workload of the flow pyramid: three Deriche "
+                        "blurs of a foam state at full size, its shrink to half size, and "
+                        "three growths of a 13-channel half-size state to full size (use "
+                        "under rocprofv3)")
     p.add_argument("--no-forms", action="store_true",
                    help="skip the precision-form step (half storage of the flow state) and the "
                         "batching-form sample (8 uneven reconstructions)")
@@ -440,6 +445,23 @@ def main():
                           "form": ctx.get_tuning("flow_variant_used"),
                           "pair_shape": ctx.get_tuning("flow_pair_used"),
                           "copy_bytes_each_way": src.numel() * 4}))
+        ctx.close()
+        return
+    if a.profile_pyramid:
+        import phantoms
+        rng = phantoms.SplitMix64(phantoms.SEED_BASE + 2)
+        vol = torch.from_numpy(phantoms.foam_phantom(L, 400, rng)).to(dev)
+        blur = torch.empty_like(vol)
+        half = torch.empty((L // 2,) * 3, device=dev)
+        state = torch.rand((13,) + (L // 2,) * 3, device=dev)
+        grown = torch.empty((13, L, L, L), device=dev)
+        for _ in range(3):
+            ctx.blur(vol, 0.8, blur)
+        ctx.resize(blur, half)
+        for _ in range(3):
+            ctx.resize(state, grown)
+        torch.cuda.synchronize()
+        print(json.dumps({"profile_pyramid": True, "size": L, "grown_channels": 13}))
         ctx.close()
         return
     if a.profile_proj:
